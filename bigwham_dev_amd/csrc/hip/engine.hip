@@ -1,0 +1,643 @@
+// MI355X (gfx950) hierarchical-matrix matvec engine and its C-ABI.
+//
+// One matvec y = H x is four kernels on the caller's stream:
+//   k_gather   x -> xt[0:n]   (permutation of matvecOriginal, hmat.cc:586-593,
+//                              or a copy; converts to the factor precision)
+//   k_slab     pass 1: t = W x   (W_k = B_k), written into xt[xt_t : ...]
+//   k_reduce   sum of split pass-1 work items (only if any)
+//   k_slab     pass 2: y = F x + U t, each y row written once, with the
+//              permutation of matvecOriginal (hmat.cc:618-626) fused in
+//   k_reduce   sum of split pass-2 work items (only if any)
+// Every byte of the factor data is streamed once per matvec with 16-byte
+// non-temporal vector loads; x and t are re-read from L2 / MALL.  See
+// layout.h for the slab format.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../../include/bigwham_hip.h"
+#include "layout.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& m) {
+  g_err = m;
+  return code;
+}
+
+#define HIPCHK(expr)                                                              \
+  do {                                                                            \
+    hipError_t e_ = (expr);                                                       \
+    if (e_ != hipSuccess)                                                         \
+      return fail(BW_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));    \
+  } while (0)
+
+typedef double dbl2 __attribute__((ext_vector_type(2)));
+typedef float flt4 __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct Vec;
+template <>
+struct Vec<double> {
+  static constexpr int V = 2;
+  typedef dbl2 type;
+};
+template <>
+struct Vec<float> {
+  static constexpr int V = 4;
+  typedef flt4 type;
+};
+
+// Workgroups are dispatched round-robin over the 8 XCDs of the MI355X; remap
+// so that each XCD runs a contiguous range of work items (neighbouring row
+// clusters share x entries in that XCD's L2).
+__device__ __forceinline__ int xcd_remap(int bid, int nblocks) {
+  const int nx = 8;
+  const int q = nblocks / nx, r = nblocks % nx;
+  const int x = bid % nx, k = bid / nx;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + k;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_gather(const double* __restrict__ x,
+                                                T* __restrict__ xt, int64_t n,
+                                                const int64_t* __restrict__ perm, int dof) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int64_t src = i;
+  if (perm) {
+    const int64_t p = i / dof;
+    src = dof * perm[p] + (i - p * dof);
+  }
+  xt[i] = static_cast<T>(x[src]);
+}
+
+template <typename OutT>
+__device__ __forceinline__ int64_t out_index(int64_t idx, const int64_t* perm, int dof) {
+  if (!perm) return idx;
+  const int64_t p = idx / dof;
+  return dof * perm[p] + (idx - p * dof);
+}
+
+// One wavefront per work item, four per workgroup.
+// T: factor / accumulation type; OutT: output type (T for pass 1, double
+// for pass 2); U: unroll depth (independent 16-byte loads in flight per lane).
+template <typename T, typename OutT, int U>
+__global__ __launch_bounds__(256) void k_slab(const bwl::WorkItem* __restrict__ wis, int nwi,
+                                              const int32_t* __restrict__ gidx,
+                                              const T* __restrict__ data,
+                                              const T* __restrict__ xin,
+                                              OutT* __restrict__ out, T* __restrict__ part,
+                                              const int64_t* __restrict__ perm, int dof) {
+  typedef typename Vec<T>::type VT;
+  constexpr int V = Vec<T>::V;
+  const int lane = threadIdx.x & 63;
+  const int w = xcd_remap(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
+  if (w >= nwi) return;
+  const bwl::WorkItem wi = wis[w];
+  const int R = wi.rows;
+  const int ng = wi.ngroups;
+  const VT* __restrict__ base = reinterpret_cast<const VT*>(data + wi.data_off);
+  const int32_t* __restrict__ gi = gidx + wi.gbeg;
+  T acc0 = 0, acc1 = 0;
+  int nsplit = 1;
+  int split = 0;
+  if (R > 32) {
+    // One column group per iteration for the whole wavefront: group index
+    // and input entries are wave-uniform (scalar loads).
+    if (lane < R) {
+      const VT* p = base + lane;
+      int g = 0;
+      for (; g + U <= ng; g += U) {
+        VT a[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) a[u] = __builtin_nontemporal_load(p + (g + u) * R);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int z = __builtin_amdgcn_readfirstlane(gi[g + u]);
+#pragma unroll
+          for (int v = 0; v < V; ++v) {
+            if (u & 1)
+              acc1 += a[u][v] * xin[z + v];
+            else
+              acc0 += a[u][v] * xin[z + v];
+          }
+        }
+      }
+      for (; g < ng; ++g) {
+        const VT a = __builtin_nontemporal_load(p + g * R);
+        const int z = __builtin_amdgcn_readfirstlane(gi[g]);
+#pragma unroll
+        for (int v = 0; v < V; ++v) acc0 += a[v] * xin[z + v];
+      }
+    }
+  } else {
+    // Several column groups per iteration: lane = split * R + row.
+    nsplit = 64 / R;
+    split = lane / R;
+    const int row = lane - split * R;
+    if (split < nsplit) {
+      const VT* p = base + row;
+      int g = split;
+      for (; g + (U - 1) * nsplit < ng; g += U * nsplit) {
+        VT a[U];
+        int z[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) a[u] = __builtin_nontemporal_load(p + (g + u * nsplit) * R);
+#pragma unroll
+        for (int u = 0; u < U; ++u) z[u] = gi[g + u * nsplit];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+#pragma unroll
+          for (int v = 0; v < V; ++v) {
+            if (u & 1)
+              acc1 += a[u][v] * xin[z[u] + v];
+            else
+              acc0 += a[u][v] * xin[z[u] + v];
+          }
+        }
+      }
+      for (; g < ng; g += nsplit) {
+        const VT a = __builtin_nontemporal_load(p + g * R);
+        const int z = gi[g];
+#pragma unroll
+        for (int v = 0; v < V; ++v) acc0 += a[v] * xin[z + v];
+      }
+    }
+  }
+  T acc = acc0 + acc1;
+  // fold the column splits onto lanes 0..R-1, in split order
+  for (int s = 1; s < nsplit; ++s) {
+    const T o = __shfl(acc, lane + s * R, 64);
+    if (split == 0) acc += o;
+  }
+  if (lane < R) {
+    if (wi.part_off >= 0) {
+      part[wi.part_off + lane] = acc;
+    } else {
+      out[out_index<OutT>(static_cast<int64_t>(wi.out_row) + lane, perm, dof)] =
+          static_cast<OutT>(acc);
+    }
+  }
+}
+
+template <typename T, typename OutT>
+__global__ __launch_bounds__(64) void k_reduce(const bwl::Reduce* __restrict__ reds, int nred,
+                                               const T* __restrict__ part, OutT* out,
+                                               const int64_t* __restrict__ perm, int dof) {
+  const int r = blockIdx.x;
+  if (r >= nred) return;
+  const bwl::Reduce rd = reds[r];
+  const int t = threadIdx.x;
+  if (t >= rd.rows) return;
+  T s = 0;
+  for (int p = 0; p < rd.nparts; ++p) s += part[rd.part_off + p * rd.rows + t];
+  out[out_index<OutT>(static_cast<int64_t>(rd.out_row) + t, perm, dof)] = static_cast<OutT>(s);
+}
+
+// ---------------------------------------------------------------------------
+
+struct DevPass {
+  bwl::WorkItem* wis = nullptr;
+  int32_t* gidx = nullptr;
+  bwl::Reduce* reds = nullptr;
+  void* data = nullptr;
+  void* part = nullptr;
+  int nwi = 0, nred = 0;
+  int64_t data_bytes = 0;
+};
+
+struct DevOp {
+  bool present = false;
+  int elem = 8;  // bytes
+  DevPass p1, p2;
+  void* xt = nullptr;
+  int64_t xt_size = 0;
+  int64_t n_in = 0, n_out = 0;
+  const int64_t* perm_in = nullptr;   // device, n_in / dof
+  const int64_t* perm_out = nullptr;  // device, n_out / dof
+  int64_t stream_bytes = 0;
+};
+
+std::atomic<int64_t> g_wi_bytes64{96 * 1024};
+std::atomic<int64_t> g_wi_bytes32{96 * 1024};
+
+template <typename T>
+int upload_pass(const bwl::Layout& L, const bwl::Program& P, DevPass* dp, hipStream_t st,
+                std::vector<void*>* allocs) {
+  dp->nwi = static_cast<int>(P.wis.size());
+  dp->nred = static_cast<int>(P.reds.size());
+  auto dalloc = [&](void** p, size_t bytes) -> int {
+    if (bytes == 0) bytes = 16;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) return fail(BW_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    allocs->push_back(*p);
+    return BW_OK;
+  };
+  int rc;
+  if ((rc = dalloc(reinterpret_cast<void**>(&dp->wis), P.wis.size() * sizeof(bwl::WorkItem))))
+    return rc;
+  if ((rc = dalloc(reinterpret_cast<void**>(&dp->gidx), P.gidx.size() * sizeof(int32_t)))) return rc;
+  if ((rc = dalloc(reinterpret_cast<void**>(&dp->reds), P.reds.size() * sizeof(bwl::Reduce))))
+    return rc;
+  if ((rc = dalloc(&dp->part, static_cast<size_t>(P.part_elems) * sizeof(T)))) return rc;
+  dp->data_bytes = P.data_elems * static_cast<int64_t>(sizeof(T));
+  if ((rc = dalloc(&dp->data, static_cast<size_t>(dp->data_bytes)))) return rc;
+  if (!P.wis.empty())
+    HIPCHK(hipMemcpy(dp->wis, P.wis.data(), P.wis.size() * sizeof(bwl::WorkItem), hipMemcpyHostToDevice));
+  if (!P.gidx.empty())
+    HIPCHK(hipMemcpy(dp->gidx, P.gidx.data(), P.gidx.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  if (!P.reds.empty())
+    HIPCHK(hipMemcpy(dp->reds, P.reds.data(), P.reds.size() * sizeof(bwl::Reduce), hipMemcpyHostToDevice));
+  // data: fill slabs into pinned staging chunks and stream them up
+  const int32_t nslab = static_cast<int32_t>(P.slabs.size());
+  if (nslab == 0) return BW_OK;
+  int64_t max_slab = 0;
+  for (const auto& s : P.slabs) max_slab = std::max<int64_t>(max_slab, s.ngroups * s.rows * L.opt.vec_elems);
+  const int64_t chunk_elems = std::max<int64_t>(max_slab, (int64_t(256) << 20) / static_cast<int64_t>(sizeof(T)));
+  T* stage[2] = {nullptr, nullptr};
+  HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&stage[0]), chunk_elems * sizeof(T)));
+  hipError_t e2 = hipHostMalloc(reinterpret_cast<void**>(&stage[1]), chunk_elems * sizeof(T));
+  if (e2 != hipSuccess) {
+    hipHostFree(stage[0]);
+    return fail(BW_ENOMEM, "hipHostMalloc staging");
+  }
+  hipEvent_t done[2];
+  hipEventCreate(&done[0]);
+  hipEventCreate(&done[1]);
+  const unsigned hc = std::max(1u, std::thread::hardware_concurrency());
+  const int nthreads = static_cast<int>(std::min(hc, 32u));
+  int buf = 0;
+  bool pending[2] = {false, false};
+  int32_t s = 0;
+  int status = BW_OK;
+  while (s < nslab) {
+    int32_t e = s;
+    int64_t elems = 0;
+    while (e < nslab) {
+      const int64_t se = P.slabs[e].ngroups * P.slabs[e].rows * L.opt.vec_elems;
+      if (elems + se > chunk_elems && e > s) break;
+      elems += se;
+      ++e;
+    }
+    if (pending[buf]) {
+      hipEventSynchronize(done[buf]);
+      pending[buf] = false;
+    }
+    T* dst = stage[buf];
+    // parallel fill over slabs of this chunk
+    std::atomic<int32_t> next{s};
+    auto worker = [&]() {
+      while (true) {
+        const int32_t i = next.fetch_add(8);
+        if (i >= e) break;
+        const int32_t j = std::min(e, i + 8);
+        bwl::fill_slabs<T>(L, P, i, j, dst + (P.slabs[i].data_off - P.slabs[s].data_off));
+      }
+    };
+    std::vector<std::thread> ths;
+    for (int t = 1; t < nthreads; ++t) ths.emplace_back(worker);
+    worker();
+    for (auto& t : ths) t.join();
+    hipError_t ce = hipMemcpyAsync(static_cast<T*>(dp->data) + P.slabs[s].data_off, dst,
+                                   elems * sizeof(T), hipMemcpyHostToDevice, st);
+    if (ce != hipSuccess) {
+      status = fail(BW_EHIP, std::string("upload: ") + hipGetErrorString(ce));
+      break;
+    }
+    hipEventRecord(done[buf], st);
+    pending[buf] = true;
+    buf ^= 1;
+    s = e;
+  }
+  hipStreamSynchronize(st);
+  hipEventDestroy(done[0]);
+  hipEventDestroy(done[1]);
+  hipHostFree(stage[0]);
+  hipHostFree(stage[1]);
+  return status;
+}
+
+}  // namespace
+
+struct bw_hip_handle {
+  int device = 0;
+  int dof = 1;
+  int64_t n_row_pts = 0, n_col_pts = 0;
+  int64_t nb_entries = 0;
+  DevOp op[2][2];  // [adjoint][fp32]
+  int64_t* perm0 = nullptr;
+  int64_t* perm1 = nullptr;
+  double* x_stage = nullptr;
+  double* y_stage = nullptr;
+  std::vector<double> diag_perm, diag_orig;
+  std::vector<void*> allocs;
+  int64_t device_bytes = 0;
+  bool timing = false;
+  hipEvent_t ev[4] = {};
+  float last_ms[4] = {0, 0, 0, 0};
+  std::mutex mu;
+};
+
+namespace {
+
+template <typename T>
+int build_op(bw_hip_handle* hd, const bw_flat_hmat* h, bool adjoint, hipStream_t st) {
+  DevOp& op = hd->op[adjoint ? 1 : 0][sizeof(T) == 4 ? 1 : 0];
+  bwl::OpView view;
+  bwl::make_view(h, adjoint, &view);
+  bwl::LayoutOptions opt;
+  opt.vec_elems = Vec<T>::V;
+  opt.elem_bytes = sizeof(T);
+  opt.wi_bytes = sizeof(T) == 8 ? g_wi_bytes64.load() : g_wi_bytes32.load();
+  bwl::Layout L;
+  bwl::build_layout(view, opt, &L);
+  int rc;
+  if ((rc = upload_pass<T>(L, L.pass1, &op.p1, st, &hd->allocs))) return rc;
+  if ((rc = upload_pass<T>(L, L.pass2, &op.p2, st, &hd->allocs))) return rc;
+  op.elem = sizeof(T);
+  op.xt_size = L.xt_size;
+  HIPCHK(hipMalloc(&op.xt, L.xt_size * sizeof(T)));
+  hd->allocs.push_back(op.xt);
+  HIPCHK(hipMemset(op.xt, 0, L.xt_size * sizeof(T)));
+  op.n_in = static_cast<int64_t>(h->dof) * view.n_col_pts;
+  op.n_out = static_cast<int64_t>(h->dof) * view.n_row_pts;
+  op.perm_in = adjoint ? hd->perm0 : hd->perm1;
+  op.perm_out = adjoint ? hd->perm1 : hd->perm0;
+  op.stream_bytes = (L.pass1.data_elems + L.pass2.data_elems) * static_cast<int64_t>(sizeof(T));
+  hd->device_bytes += op.p1.data_bytes + op.p2.data_bytes + L.xt_size * sizeof(T) +
+                      (L.pass1.part_elems + L.pass2.part_elems) * sizeof(T) +
+                      (L.pass1.wis.size() + L.pass2.wis.size()) * sizeof(bwl::WorkItem) +
+                      (L.pass1.gidx.size() + L.pass2.gidx.size()) * sizeof(int32_t);
+  op.present = true;
+  return BW_OK;
+}
+
+void compute_diagonal(bw_hip_handle* hd, const bw_flat_hmat* h) {
+  const int64_t d = h->dof;
+  const int64_t npt = std::max(h->n_row_pts, h->n_col_pts);
+  hd->diag_perm.assign(d * npt, 0.0);
+  for (int64_t k = 0; k < h->n_fr; ++k) {
+    const int64_t* c = h->fr_pattern + 6 * k;
+    const int64_t i0 = c[1], j0 = c[2], iend = c[3], jend = c[4];
+    const bool lower = (i0 > j0) && (i0 > jend) && (iend > j0) && (iend > jend);
+    const bool upper = (i0 < j0) && (i0 < jend) && (iend < j0) && (iend < jend);
+    if (lower || upper) continue;
+    const int64_t m = d * (iend - i0), n = d * (jend - j0);
+    const double* F = h->fr_blocks[k];
+    for (int64_t j = 0; j < n; ++j) {
+      const int64_t i = j + d * j0 - d * i0;
+      if (i >= 0 && i < m) hd->diag_perm[i + d * i0] = F[i + j * m];
+    }
+  }
+  hd->diag_orig.assign(d * npt, 0.0);
+  for (int64_t i = 0; i < npt && i < h->n_col_pts; ++i)
+    for (int64_t j = 0; j < d; ++j) hd->diag_orig[d * h->perm1[i] + j] = hd->diag_perm[d * i + j];
+}
+
+template <typename T>
+int launch_matvec(bw_hip_handle* hd, DevOp& op, const double* xd, double* yd, int ordering,
+                  hipStream_t st) {
+  const bool orig = ordering == BW_ORDER_ORIGINAL;
+  const int dof = hd->dof;
+  T* xt = static_cast<T*>(op.xt);
+  if (hd->timing) hipEventRecord(hd->ev[0], st);
+  {
+    const int64_t n = op.n_in;
+    const int blocks = static_cast<int>((n + 255) / 256);
+    hipLaunchKernelGGL(k_gather<T>, dim3(blocks), dim3(256), 0, st, xd, xt, n,
+                       orig ? op.perm_in : nullptr, dof);
+  }
+  if (hd->timing) hipEventRecord(hd->ev[1], st);
+  if (op.p1.nwi > 0) {
+    const int blocks = (op.p1.nwi + 3) / 4;
+    hipLaunchKernelGGL((k_slab<T, T, 8>), dim3(blocks), dim3(256), 0, st, op.p1.wis, op.p1.nwi,
+                       op.p1.gidx, static_cast<const T*>(op.p1.data), xt, xt,
+                       static_cast<T*>(op.p1.part), nullptr, dof);
+    if (op.p1.nred > 0)
+      hipLaunchKernelGGL((k_reduce<T, T>), dim3(op.p1.nred), dim3(64), 0, st, op.p1.reds,
+                         op.p1.nred, static_cast<const T*>(op.p1.part), xt, nullptr, dof);
+  }
+  if (hd->timing) hipEventRecord(hd->ev[2], st);
+  if (op.p2.nwi > 0) {
+    const int blocks = (op.p2.nwi + 3) / 4;
+    hipLaunchKernelGGL((k_slab<T, double, 8>), dim3(blocks), dim3(256), 0, st, op.p2.wis,
+                       op.p2.nwi, op.p2.gidx, static_cast<const T*>(op.p2.data), xt, yd,
+                       static_cast<T*>(op.p2.part), orig ? op.perm_out : nullptr, dof);
+    if (op.p2.nred > 0)
+      hipLaunchKernelGGL((k_reduce<T, double>), dim3(op.p2.nred), dim3(64), 0, st, op.p2.reds,
+                         op.p2.nred, static_cast<const T*>(op.p2.part), yd,
+                         orig ? op.perm_out : nullptr, dof);
+  }
+  if (hd->timing) hipEventRecord(hd->ev[3], st);
+  HIPCHK(hipGetLastError());
+  return BW_OK;
+}
+
+int do_matvec(bw_hip_handle* hd, bool adjoint, const double* x, double* y, int ordering,
+              int precision, int on_device, void* stream) {
+  if (!hd || !x || !y) return fail(BW_EINVAL, "matvec: null argument");
+  if (ordering != BW_ORDER_PERMUTED && ordering != BW_ORDER_ORIGINAL)
+    return fail(BW_EINVAL, "matvec: ordering must be 0 (permuted) or 1 (original)");
+  if (precision != BW_PREC_FP64 && precision != BW_PREC_FP32)
+    return fail(BW_EINVAL, "matvec: precision must be BW_PREC_FP64 or BW_PREC_FP32");
+  DevOp& op = hd->op[adjoint ? 1 : 0][precision == BW_PREC_FP32 ? 1 : 0];
+  if (!op.present)
+    return fail(BW_ENOTSUP, adjoint ? "rmatvec: adjoint / precision not built for this handle"
+                                    : "matvec: precision not built for this handle");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  HIPCHK(hipSetDevice(hd->device));
+  const double* xd = x;
+  double* yd = y;
+  if (!on_device) {
+    HIPCHK(hipMemcpyAsync(hd->x_stage, x, op.n_in * sizeof(double), hipMemcpyHostToDevice, st));
+    xd = hd->x_stage;
+    yd = hd->y_stage;
+  }
+  int rc = precision == BW_PREC_FP64 ? launch_matvec<double>(hd, op, xd, yd, ordering, st)
+                                     : launch_matvec<float>(hd, op, xd, yd, ordering, st);
+  if (rc) return rc;
+  if (!on_device) {
+    HIPCHK(hipMemcpyAsync(y, hd->y_stage, op.n_out * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
+  if (hd->timing) {
+    HIPCHK(hipEventSynchronize(hd->ev[3]));
+    hipEventElapsedTime(&hd->last_ms[0], hd->ev[0], hd->ev[1]);
+    hipEventElapsedTime(&hd->last_ms[1], hd->ev[1], hd->ev[2]);
+    hipEventElapsedTime(&hd->last_ms[2], hd->ev[2], hd->ev[3]);
+    hipEventElapsedTime(&hd->last_ms[3], hd->ev[0], hd->ev[3]);
+  }
+  return BW_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* bw_hip_last_error(void) { return g_err.c_str(); }
+
+int bw_hip_set_tuning(int64_t wi64, int64_t wi32) {
+  if (wi64 < 0 || wi32 < 0) return fail(BW_EINVAL, "set_tuning: negative size");
+  g_wi_bytes64 = wi64 ? wi64 : 96 * 1024;
+  g_wi_bytes32 = wi32 ? wi32 : 96 * 1024;
+  return BW_OK;
+}
+
+void bw_hip_destroy(bw_hip_handle* hd) {
+  if (!hd) return;
+  hipSetDevice(hd->device);
+  hipDeviceSynchronize();
+  for (void* p : hd->allocs) hipFree(p);
+  for (auto& e : hd->ev)
+    if (e) hipEventDestroy(e);
+  delete hd;
+}
+
+int bw_hip_create(const bw_flat_hmat* h, int precision_mask, int flags, int device,
+                  bw_hip_handle** out) {
+  if (!out) return fail(BW_EINVAL, "create: null output pointer");
+  *out = nullptr;
+  std::string msg;
+  int rc = bwl::validate_flat(h, &msg);
+  if (rc) return fail(rc, "create: " + msg);
+  if ((precision_mask & (BW_PREC_FP64 | BW_PREC_FP32)) == 0 ||
+      (precision_mask & ~(BW_PREC_FP64 | BW_PREC_FP32)))
+    return fail(BW_EINVAL, "create: precision_mask must combine BW_PREC_FP64 / BW_PREC_FP32");
+  if (flags & ~BW_FLAG_ADJOINT) return fail(BW_EINVAL, "create: unknown flag");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return fail(BW_EHIP, "create: no HIP device available");
+  auto* hd = new bw_hip_handle();
+  if (device < 0) {
+    if (hipGetDevice(&hd->device) != hipSuccess) hd->device = 0;
+  } else {
+    hd->device = device;
+  }
+  auto cleanup = [&](int code) {
+    bw_hip_destroy(hd);
+    return code;
+  };
+  if (hipSetDevice(hd->device) != hipSuccess) return cleanup(fail(BW_EHIP, "create: hipSetDevice"));
+  hd->dof = h->dof;
+  hd->n_row_pts = h->n_row_pts;
+  hd->n_col_pts = h->n_col_pts;
+  const int64_t d = h->dof;
+  auto dmal = [&](void** p, size_t bytes) -> int {
+    hipError_t e = hipMalloc(p, std::max<size_t>(bytes, 16));
+    if (e != hipSuccess) return fail(BW_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    hd->allocs.push_back(*p);
+    return BW_OK;
+  };
+  if ((rc = dmal(reinterpret_cast<void**>(&hd->perm0), h->n_row_pts * sizeof(int64_t)))) return cleanup(rc);
+  if ((rc = dmal(reinterpret_cast<void**>(&hd->perm1), h->n_col_pts * sizeof(int64_t)))) return cleanup(rc);
+  const int64_t nmax = d * std::max(h->n_row_pts, h->n_col_pts);
+  if ((rc = dmal(reinterpret_cast<void**>(&hd->x_stage), nmax * sizeof(double)))) return cleanup(rc);
+  if ((rc = dmal(reinterpret_cast<void**>(&hd->y_stage), nmax * sizeof(double)))) return cleanup(rc);
+  if (hipMemcpy(hd->perm0, h->perm0, h->n_row_pts * sizeof(int64_t), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(hd->perm1, h->perm1, h->n_col_pts * sizeof(int64_t), hipMemcpyHostToDevice) != hipSuccess)
+    return cleanup(fail(BW_EHIP, "create: permutation upload"));
+  hipStream_t st;
+  if (hipStreamCreate(&st) != hipSuccess) return cleanup(fail(BW_EHIP, "create: hipStreamCreate"));
+  for (int adj = 0; adj < 2; ++adj) {
+    if (adj && !(flags & BW_FLAG_ADJOINT)) continue;
+    if (precision_mask & BW_PREC_FP64) {
+      rc = build_op<double>(hd, h, adj != 0, st);
+      if (rc) {
+        hipStreamDestroy(st);
+        return cleanup(rc);
+      }
+    }
+    if (precision_mask & BW_PREC_FP32) {
+      rc = build_op<float>(hd, h, adj != 0, st);
+      if (rc) {
+        hipStreamDestroy(st);
+        return cleanup(rc);
+      }
+    }
+  }
+  hipStreamDestroy(st);
+  // nbOfEntries (hmat.cc:299-312)
+  int64_t E = 0;
+  for (int64_t k = 0; k < h->n_fr; ++k) {
+    const int64_t* c = h->fr_pattern + 6 * k;
+    E += d * (c[3] - c[1]) * d * (c[4] - c[2]);
+  }
+  for (int64_t k = 0; k < h->n_lr; ++k) {
+    const int64_t* c = h->lr_pattern + 6 * k;
+    E += d * (c[3] - c[1]) * c[5] + c[5] * d * (c[4] - c[2]);
+  }
+  hd->nb_entries = E;
+  compute_diagonal(hd, h);
+  for (auto& e : hd->ev)
+    if (hipEventCreate(&e) != hipSuccess) return cleanup(fail(BW_EHIP, "create: hipEventCreate"));
+  *out = hd;
+  return BW_OK;
+}
+
+int bw_hip_matvec(bw_hip_handle* hd, const double* x, double* y, int ordering, int precision,
+                  int on_device, void* stream) {
+  return do_matvec(hd, false, x, y, ordering, precision, on_device, stream);
+}
+
+int bw_hip_rmatvec(bw_hip_handle* hd, const double* x, double* y, int ordering, int precision,
+                   int on_device, void* stream) {
+  return do_matvec(hd, true, x, y, ordering, precision, on_device, stream);
+}
+
+int bw_hip_matvec_batched(bw_hip_handle* hd, int nrhs, const double* X, int64_t ldx, double* Y,
+                          int64_t ldy, int ordering, int precision, void* stream) {
+  if (!hd || !X || !Y || nrhs <= 0) return fail(BW_EINVAL, "matvec_batched: bad argument");
+  const DevOp& op = hd->op[0][precision == BW_PREC_FP32 ? 1 : 0];
+  if (!op.present) return fail(BW_ENOTSUP, "matvec_batched: precision not built");
+  if (ldx < op.n_in || ldy < op.n_out) return fail(BW_ESHAPE, "matvec_batched: leading dimension too small");
+  for (int k = 0; k < nrhs; ++k) {
+    int rc = do_matvec(hd, false, X + k * ldx, Y + k * ldy, ordering, precision, 1, stream);
+    if (rc) return rc;
+  }
+  return BW_OK;
+}
+
+int bw_hip_diagonal(bw_hip_handle* hd, double* dg, int ordering) {
+  if (!hd || !dg) return fail(BW_EINVAL, "diagonal: null argument");
+  if (ordering != BW_ORDER_PERMUTED && ordering != BW_ORDER_ORIGINAL)
+    return fail(BW_EINVAL, "diagonal: bad ordering");
+  const auto& v = ordering == BW_ORDER_ORIGINAL ? hd->diag_orig : hd->diag_perm;
+  std::memcpy(dg, v.data(), v.size() * sizeof(double));
+  return BW_OK;
+}
+
+int bw_hip_sizes(bw_hip_handle* hd, int64_t* n_rows, int64_t* n_cols, int64_t* nb_entries,
+                 int64_t* device_bytes, int64_t* sb64, int64_t* sb32) {
+  if (!hd) return fail(BW_EINVAL, "sizes: null handle");
+  if (n_rows) *n_rows = hd->dof * hd->n_row_pts;
+  if (n_cols) *n_cols = hd->dof * hd->n_col_pts;
+  if (nb_entries) *nb_entries = hd->nb_entries;
+  if (device_bytes) *device_bytes = hd->device_bytes;
+  if (sb64) *sb64 = hd->op[0][0].present ? hd->op[0][0].stream_bytes : 0;
+  if (sb32) *sb32 = hd->op[0][1].present ? hd->op[0][1].stream_bytes : 0;
+  return BW_OK;
+}
+
+int bw_hip_set_timing(bw_hip_handle* hd, int enable) {
+  if (!hd) return fail(BW_EINVAL, "set_timing: null handle");
+  hd->timing = enable != 0;
+  return BW_OK;
+}
+
+int bw_hip_last_timing(bw_hip_handle* hd, float* ms4) {
+  if (!hd || !ms4) return fail(BW_EINVAL, "last_timing: null argument");
+  std::memcpy(ms4, hd->last_ms, sizeof(hd->last_ms));
+  return BW_OK;
+}
+
+}  // extern "C"

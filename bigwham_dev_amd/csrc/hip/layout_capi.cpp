@@ -1,0 +1,92 @@
+// Host-only introspection of the slab layout (no GPU needed): lets tests
+// check every work item's bounds and replay the two passes on the CPU before
+// a kernel ever runs.
+#include <cstring>
+#include <string>
+
+#include "../../../include/bigwham_hip.h"
+#include "layout.h"
+
+namespace {
+thread_local std::string g_lerr;
+}
+
+struct bw_layout {
+  bwl::Layout L;
+};
+
+extern "C" {
+
+const char* bw_layout_last_error(void) { return g_lerr.c_str(); }
+
+int bw_layout_build(const bw_flat_hmat* h, int precision, int adjoint, int64_t wi_bytes,
+                    bw_layout** out) {
+  if (!out) return BW_EINVAL;
+  std::string msg;
+  int rc = bwl::validate_flat(h, &msg);
+  if (rc) {
+    g_lerr = msg;
+    return rc;
+  }
+  if (precision != BW_PREC_FP64 && precision != BW_PREC_FP32) {
+    g_lerr = "precision";
+    return BW_EINVAL;
+  }
+  auto* lay = new bw_layout();
+  bwl::OpView v;
+  bwl::make_view(h, adjoint != 0, &v);
+  bwl::LayoutOptions opt;
+  opt.vec_elems = precision == BW_PREC_FP64 ? 2 : 4;
+  opt.elem_bytes = precision == BW_PREC_FP64 ? 8 : 4;
+  if (wi_bytes > 0) opt.wi_bytes = wi_bytes;
+  bwl::build_layout(v, opt, &lay->L);
+  *out = lay;
+  return BW_OK;
+}
+
+// info[0..1] = xt_t, xt_size; info[2+6p .. 7+6p] for pass p (0, 1) =
+// n_wi, n_gidx, n_red, data_elems, part_elems, out_rows; info[14] = V.
+int bw_layout_info(const bw_layout* lay, int64_t* info) {
+  if (!lay || !info) return BW_EINVAL;
+  info[0] = lay->L.xt_t;
+  info[1] = lay->L.xt_size;
+  const bwl::Program* P[2] = {&lay->L.pass1, &lay->L.pass2};
+  for (int p = 0; p < 2; ++p) {
+    info[2 + 6 * p] = static_cast<int64_t>(P[p]->wis.size());
+    info[3 + 6 * p] = static_cast<int64_t>(P[p]->gidx.size());
+    info[4 + 6 * p] = static_cast<int64_t>(P[p]->reds.size());
+    info[5 + 6 * p] = P[p]->data_elems;
+    info[6 + 6 * p] = P[p]->part_elems;
+    info[7 + 6 * p] = P[p]->out_rows;
+  }
+  info[14] = lay->L.opt.vec_elems;
+  return BW_OK;
+}
+
+// wis: 8 int64 per WI (data_off, gbeg, ngroups, rows, out_row, part_off, 0, 0);
+// reds: 4 int64 per reduce; data in double regardless of precision.
+int bw_layout_export(const bw_layout* lay, int pass, int64_t* wis, int32_t* gidx, int64_t* reds,
+                     double* data) {
+  if (!lay || (pass != 1 && pass != 2)) return BW_EINVAL;
+  const bwl::Program& P = pass == 1 ? lay->L.pass1 : lay->L.pass2;
+  if (wis)
+    for (size_t i = 0; i < P.wis.size(); ++i) {
+      const bwl::WorkItem& w = P.wis[i];
+      int64_t* o = wis + 8 * i;
+      o[0] = w.data_off; o[1] = w.gbeg; o[2] = w.ngroups; o[3] = w.rows;
+      o[4] = w.out_row; o[5] = w.part_off; o[6] = 0; o[7] = 0;
+    }
+  if (gidx && !P.gidx.empty()) std::memcpy(gidx, P.gidx.data(), P.gidx.size() * sizeof(int32_t));
+  if (reds)
+    for (size_t i = 0; i < P.reds.size(); ++i) {
+      reds[4 * i] = P.reds[i].out_row; reds[4 * i + 1] = P.reds[i].rows;
+      reds[4 * i + 2] = P.reds[i].part_off; reds[4 * i + 3] = P.reds[i].nparts;
+    }
+  if (data && !P.slabs.empty())
+    bwl::fill_slabs<double>(lay->L, P, 0, static_cast<int32_t>(P.slabs.size()), data);
+  return BW_OK;
+}
+
+void bw_layout_free(bw_layout* lay) { delete lay; }
+
+}  // extern "C"

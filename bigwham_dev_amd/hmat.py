@@ -1,0 +1,276 @@
+"""Python handles over the two native libraries.
+
+HostHmat  — host-side construction (libbwhost.so): mesh, kernel, cluster tree,
+            pattern, full-rank blocks and ACA low-rank blocks; mirrors
+            BigWhamIO's constructor / BuildPattern / BuildHierarchicalMatrix
+            (src/io/bigwham_io.cc:41-290, :487-512, :635-704).
+HipHmat   — the MI355X engine (libbwhip.so): device-resident layout and the
+            matvec / adjoint / batched entry points (drop-in for
+            Hmat::matvec / matvecOriginal, src/hmat/hmatrix/hmat.cc:472-715,
+            and HmatCuda::matvec, hmat_cuda.cc:1771-2460).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _native as N
+
+
+class HostHmat:
+    """Host-side H-matrix (owned by libbwhost)."""
+
+    def __init__(self, handle: int, keepalive=None):
+        self._h = C.c_void_p(handle)
+        self._keep = keepalive
+        self._flat = None
+
+    # ---- construction -------------------------------------------------
+    @classmethod
+    def from_mesh(cls, kernel: str, coor, conn, properties, n_threads: int = -1,
+                  homogeneous_size_pattern: bool = False, fixed_rank: int = -1) -> "HostHmat":
+        coor = np.ascontiguousarray(coor, dtype=np.float64)
+        conn = np.ascontiguousarray(conn, dtype=np.int64)
+        props = np.ascontiguousarray(np.asarray(properties, dtype=np.float64).ravel())
+        out = C.c_void_p()
+        N.check_host(N.host().bwh_create(kernel.encode(), N.dptr(coor), coor.shape[0],
+                                         N.iptr(conn), conn.shape[0], N.dptr(props),
+                                         props.size, int(n_threads),
+                                         int(bool(homogeneous_size_pattern)), int(fixed_rank),
+                                         C.byref(out)))
+        return cls(out.value)
+
+    @classmethod
+    def gaussian(cls, n: int, alpha: float, n_threads: int = -1,
+                 homogeneous_size_pattern: bool = True) -> "HostHmat":
+        out = C.c_void_p()
+        N.check_host(N.host().bwh_create_gaussian(int(n), float(alpha), int(n_threads),
+                                                  int(bool(homogeneous_size_pattern)),
+                                                  C.byref(out)))
+        return cls(out.value)
+
+    def build_pattern(self, max_leaf_size: int, eta: float) -> None:
+        N.check_host(N.host().bwh_build_pattern(self._h, int(max_leaf_size), float(eta)))
+        self._flat = None
+
+    def build(self, max_leaf_size: int, eta: float, eps_aca: float) -> None:
+        N.check_host(N.host().bwh_build_hmat(self._h, int(max_leaf_size), float(eta),
+                                             float(eps_aca)))
+        self._flat = None
+
+    def __del__(self):
+        try:
+            if self._h:
+                N.host().bwh_free(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    # ---- views ----------------------------------------------------------
+    def flat(self) -> N.FlatHmat:
+        if self._flat is None:
+            f = N.FlatHmat()
+            N.check_host(N.host().bwh_flat(self._h, C.byref(f)))
+            self._flat = f
+        return self._flat
+
+    def info(self):
+        dof, dim, n = C.c_int(), C.c_int(), C.c_int64()
+        N.check_host(N.host().bwh_info(self._h, C.byref(dof), C.byref(dim), C.byref(n)))
+        return dof.value, dim.value, n.value
+
+    @property
+    def dof(self) -> int:
+        return self.info()[0]
+
+    @property
+    def size(self) -> int:
+        dof, _, n = self.info()
+        return dof * n
+
+    def collocation_points(self) -> np.ndarray:
+        dof, dim, n = self.info()
+        out = np.empty((n, dim))
+        N.check_host(N.host().bwh_collocation_points(self._h, N.dptr(out)))
+        return out
+
+    def permutation(self) -> np.ndarray:
+        f = self.flat()
+        return np.ctypeslib.as_array(f.perm1, shape=(f.n_col_pts,)).copy()
+
+    def fr_pattern(self) -> np.ndarray:
+        f = self.flat()
+        if f.n_fr == 0:
+            return np.zeros((6, 0), dtype=np.int64)
+        return np.ctypeslib.as_array(f.fr_pattern, shape=(f.n_fr, 6)).T.copy()
+
+    def lr_pattern(self) -> np.ndarray:
+        f = self.flat()
+        if f.n_lr == 0:
+            return np.zeros((6, 0), dtype=np.int64)
+        return np.ctypeslib.as_array(f.lr_pattern, shape=(f.n_lr, 6)).T.copy()
+
+    def fr_block(self, k: int) -> np.ndarray:
+        f = self.flat()
+        p = self.fr_pattern()[:, k]
+        d = f.dof
+        m, n = d * (p[3] - p[1]), d * (p[4] - p[2])
+        return np.ctypeslib.as_array(f.fr_blocks[k], shape=(n, m)).T.copy()
+
+    def lr_block(self, k: int):
+        f = self.flat()
+        p = self.lr_pattern()[:, k]
+        d = f.dof
+        m, n, r = d * (p[3] - p[1]), d * (p[4] - p[2]), p[5]
+        if r == 0:
+            return np.zeros((m, 0)), np.zeros((0, n))
+        A = np.ctypeslib.as_array(f.lr_A[k], shape=(r, m)).T.copy()
+        B = np.ctypeslib.as_array(f.lr_B[k], shape=(n, r)).T.copy()
+        return A, B
+
+    def nb_entries(self) -> int:
+        d = self.dof
+        fp, lp = self.fr_pattern(), self.lr_pattern()
+        e = int(np.sum(d * (fp[3] - fp[1]) * d * (fp[4] - fp[2])))
+        e += int(np.sum(d * (lp[3] - lp[1]) * lp[5] + lp[5] * d * (lp[4] - lp[2])))
+        return e
+
+    def compression_ratio(self) -> float:
+        n = self.size
+        return self.nb_entries() / float(n * n)
+
+    def dense_block(self, b0: int, b1: int, m: int, n: int) -> np.ndarray:
+        d = self.dof
+        out = np.empty((d * n, d * m))  # column-major (d*m) x (d*n) seen transposed
+        N.check_host(N.host().bwh_dense_block(self._h, b0, b1, m, n, N.dptr(out)))
+        return out.T
+
+    def dense(self, original: bool = False) -> np.ndarray:
+        """Exact operator (toArray2D, matrix_generator.h:35-40), permuted or original order."""
+        dof, _, n = self.info()
+        M = self.dense_block(0, 0, n, n)
+        if not original:
+            return M
+        perm = self.permutation()
+        idx = (dof * perm[:, None] + np.arange(dof)[None, :]).ravel()
+        out = np.empty_like(M)
+        out[np.ix_(idx, idx)] = M
+        return out
+
+    def influence(self, e_src: int, e_rec: int) -> np.ndarray:
+        d = self.dof
+        out = np.empty(d * d)
+        N.check_host(N.host().bwh_influence(self._h, e_src, e_rec, N.dptr(out)))
+        return out.reshape(d, d).T
+
+    def max_aca_error(self) -> float:
+        v = C.c_double()
+        N.check_host(N.host().bwh_max_aca_error(self._h, C.byref(v)))
+        return v.value
+
+    def timings(self):
+        a, b = C.c_double(), C.c_double()
+        N.check_host(N.host().bwh_timings(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+
+class HipHmat:
+    """Device-resident H-matrix on one MI355X (owned by libbwhip)."""
+
+    def __init__(self, host: HostHmat, precisions=("fp64",), adjoint: bool = False,
+                 device: int = -1):
+        mask = 0
+        for p in precisions:
+            mask |= {"fp64": N.BW_PREC_FP64, "fp32": N.BW_PREC_FP32}[p]
+        flags = N.BW_FLAG_ADJOINT if adjoint else 0
+        out = C.c_void_p()
+        self._host = host  # host arrays may be freed after create; kept for convenience
+        N.check_hip(N.hip().bw_hip_create(C.byref(host.flat()), mask, flags, int(device),
+                                          C.byref(out)))
+        self._h = out
+        self.precisions = tuple(precisions)
+        self.adjoint = adjoint
+        r, c, e, db, s64, s32 = (C.c_int64() for _ in range(6))
+        N.check_hip(N.hip().bw_hip_sizes(self._h, C.byref(r), C.byref(c), C.byref(e),
+                                         C.byref(db), C.byref(s64), C.byref(s32)))
+        self.n_rows, self.n_cols = r.value, c.value
+        self.nb_entries = e.value
+        self.device_bytes = db.value
+        self.stream_bytes = {"fp64": s64.value, "fp32": s32.value}
+
+    def __del__(self):
+        try:
+            if self._h:
+                N.hip().bw_hip_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    @staticmethod
+    def _prec(p: str) -> int:
+        return {"fp64": N.BW_PREC_FP64, "fp32": N.BW_PREC_FP32}[p]
+
+    def matvec_ptr(self, x_ptr: int, y_ptr: int, original: bool = True, precision="fp64",
+                   stream: int = 0, adjoint: bool = False) -> None:
+        fn = N.hip().bw_hip_rmatvec if adjoint else N.hip().bw_hip_matvec
+        N.check_hip(fn(self._h, C.c_void_p(x_ptr), C.c_void_p(y_ptr),
+                       N.BW_ORDER_ORIGINAL if original else N.BW_ORDER_PERMUTED,
+                       self._prec(precision), 1, C.c_void_p(stream)))
+
+    def matvec(self, x, original: bool = True, precision: str = "fp64", out=None,
+               adjoint: bool = False):
+        """y = H x (or H^T x).  numpy in -> numpy out (host staging, like
+        HmatCuda::matvec(ArrayView)); torch CUDA tensor in -> tensor out."""
+        if isinstance(x, np.ndarray):
+            x = np.ascontiguousarray(x, dtype=np.float64)
+            n_out = self.n_cols if adjoint else self.n_rows
+            y = np.empty(n_out) if out is None else out
+            fn = N.hip().bw_hip_rmatvec if adjoint else N.hip().bw_hip_matvec
+            N.check_hip(fn(self._h, x.ctypes.data_as(C.c_void_p), y.ctypes.data_as(C.c_void_p),
+                           N.BW_ORDER_ORIGINAL if original else N.BW_ORDER_PERMUTED,
+                           self._prec(precision), 0, None))
+            return y
+        import torch
+        if not (isinstance(x, torch.Tensor) and x.is_cuda and x.dtype == torch.float64):
+            raise TypeError("x must be a numpy array or a float64 CUDA tensor")
+        x = x.contiguous()
+        n_out = self.n_cols if adjoint else self.n_rows
+        y = torch.empty(n_out, dtype=torch.float64, device=x.device) if out is None else out
+        stream = torch.cuda.current_stream(x.device).cuda_stream
+        self.matvec_ptr(x.data_ptr(), y.data_ptr(), original, precision, stream, adjoint)
+        return y
+
+    def rmatvec(self, x, original: bool = True, precision: str = "fp64", out=None):
+        return self.matvec(x, original, precision, out, adjoint=True)
+
+    def matvec_batched(self, X, original: bool = True, precision: str = "fp64", out=None):
+        """Y = H X for a (n, k) float64 CUDA tensor (column-major storage: X.T contiguous)."""
+        import torch
+        Xc = X.t().contiguous()  # k x n row-major == n x k column-major
+        k = Xc.shape[0]
+        Y = torch.empty((k, self.n_rows), dtype=torch.float64, device=X.device)
+        stream = torch.cuda.current_stream(X.device).cuda_stream
+        N.check_hip(N.hip().bw_hip_matvec_batched(
+            self._h, k, C.c_void_p(Xc.data_ptr()), self.n_cols, C.c_void_p(Y.data_ptr()),
+            self.n_rows, N.BW_ORDER_ORIGINAL if original else N.BW_ORDER_PERMUTED,
+            self._prec(precision), C.c_void_p(stream)))
+        return Y.t()
+
+    def diagonal(self, original: bool = True) -> np.ndarray:
+        d = np.empty(max(self.n_rows, self.n_cols))
+        N.check_hip(N.hip().bw_hip_diagonal(self._h, N.dptr(d),
+                                            N.BW_ORDER_ORIGINAL if original else N.BW_ORDER_PERMUTED))
+        return d
+
+    def set_timing(self, enable: bool = True) -> None:
+        N.check_hip(N.hip().bw_hip_set_timing(self._h, int(enable)))
+
+    def last_timing(self):
+        ms = (C.c_float * 4)()
+        N.check_hip(N.hip().bw_hip_last_timing(self._h, ms))
+        return list(ms)
+
+
+def set_tuning(wi_bytes_fp64: int = 0, wi_bytes_fp32: int = 0) -> None:
+    N.check_hip(N.hip().bw_hip_set_tuning(int(wi_bytes_fp64), int(wi_bytes_fp32)))

@@ -1,0 +1,112 @@
+/*
+ * bigwham_hip.h — C-ABI of the MI355X hierarchical-matrix engine
+ * (libbwhip.so).  Drop-in for the matvec path of the reference:
+ *
+ *   Hmat<T>::matvec(il::ArrayView<T>)          src/hmat/hmatrix/hmat.cc:472-537
+ *   Hmat<T>::matvecOriginal(...) (4 overloads) src/hmat/hmatrix/hmat.cc:572-715
+ *   HmatCuda<double>::matvec()                 src/hmat/hmatrix/hmat_cuda.cc:1771-2346
+ *   HmatCuda::matvec(double*, double*) / matvecOriginal(double*, double*)
+ *                                              src/hmat/hmatrix/hmat_cuda.cc:2397-2460
+ *   BigWhamIO::MatVec / MatVecPerm / MatVecVoid src/io/bigwham_io.cc:919-980
+ *   Hmat<T>::diagonal / diagonalOriginal       src/hmat/hmatrix/hmat.cc:237-285
+ *   Hmat<T>::nbOfEntries                       src/hmat/hmatrix/hmat.cc:299-312
+ *
+ * The handle owns device copies of the blocks, laid out once at creation.
+ * All vectors are caller-owned.  Calls are asynchronous on the given HIP
+ * stream when the vectors are device pointers; with host pointers the call
+ * copies in, computes and copies out before returning (like
+ * HmatCuda::matvec(il::ArrayView), hmat_cuda.cc:2349-2394).  Calls on one
+ * handle share its scratch vectors and must not overlap.
+ * Every function returns BW_OK (0) or a negative BW_E* code and never exits
+ * the process; bw_hip_last_error() gives the message of the calling thread's
+ * last failure.
+ */
+#ifndef BIGWHAM_HIP_H
+#define BIGWHAM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "bigwham_flat.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct bw_hip_handle bw_hip_handle;
+
+/* precision_mask bits: which factor copies to keep on the device */
+#define BW_PREC_FP64 1
+#define BW_PREC_FP32 2
+/* flags */
+#define BW_FLAG_ADJOINT 1 /* also lay out H^T for bw_hip_rmatvec */
+
+/* ordering: BW_ORDER_PERMUTED is Hmat::matvec / BigWhamIO::MatVecPerm;
+ * BW_ORDER_ORIGINAL is Hmat::matvecOriginal / BigWhamIO::MatVec. */
+#define BW_ORDER_PERMUTED 0
+#define BW_ORDER_ORIGINAL 1
+
+/* Builds the device layout and uploads it (replaces the HmatCuda constructor
+ * and copyToDevice, hmat_cuda.cc:98-1183).  The host arrays of `h` may be
+ * freed after return.  device: HIP device ordinal (-1: current). */
+int bw_hip_create(const bw_flat_hmat* h, int precision_mask, int flags, int device,
+                  bw_hip_handle** out);
+
+/* y = H x.  precision: BW_PREC_FP64 (FP64 factors, FP64 accumulation) or
+ * BW_PREC_FP32 (FP32 factors, FP32 accumulation; x, y stay double).
+ * on_device: 1 if x and y are device pointers, 0 for host pointers.
+ * stream: hipStream_t or NULL for the null stream. */
+int bw_hip_matvec(bw_hip_handle* hd, const double* x, double* y, int ordering,
+                  int precision, int on_device, void* stream);
+
+/* y = H^T x (requires BW_FLAG_ADJOINT at creation). */
+int bw_hip_rmatvec(bw_hip_handle* hd, const double* x, double* y, int ordering,
+                   int precision, int on_device, void* stream);
+
+/* Y = H X for nrhs right-hand sides.  X is (n_cols x nrhs) column-major with
+ * leading dimension ldx, Y (n_rows x nrhs) with ldy; device pointers only.
+ * FP32 runs the pass on MFMA thin-GEMM tiles; FP64 on the streaming path. */
+int bw_hip_matvec_batched(bw_hip_handle* hd, int nrhs, const double* X, int64_t ldx,
+                          double* Y, int64_t ldy, int ordering, int precision,
+                          void* stream);
+
+/* Diagonal (length n_rows), from the full-rank blocks that cross the
+ * diagonal; ordering as above.  Host pointer. */
+int bw_hip_diagonal(bw_hip_handle* hd, double* d, int ordering);
+
+/* Sizes: rows / cols in DoF, nbOfEntries (algorithmic entries E), device
+ * bytes held by the handle, bytes streamed by one FP64 / FP32 matvec. */
+int bw_hip_sizes(bw_hip_handle* hd, int64_t* n_rows, int64_t* n_cols,
+                 int64_t* nb_entries, int64_t* device_bytes,
+                 int64_t* stream_bytes_fp64, int64_t* stream_bytes_fp32);
+
+/* Tuning knobs for the next create (work-item target bytes; 0 = default). */
+int bw_hip_set_tuning(int64_t wi_bytes_fp64, int64_t wi_bytes_fp32);
+
+/* Per-kernel timing of the last matvec issued with timing enabled:
+ * enable != 0 records HIP events around each pass on the call's stream;
+ * ms[0..3] = gather, pass 1, pass 2 (incl. reduce), total. */
+int bw_hip_set_timing(bw_hip_handle* hd, int enable);
+int bw_hip_last_timing(bw_hip_handle* hd, float* ms4);
+
+void bw_hip_destroy(bw_hip_handle* hd);
+const char* bw_hip_last_error(void);
+
+/* Host-only introspection of the slab layout (no GPU needed; used by tests to
+ * check every work item's bounds and to replay the passes on the CPU).
+ * No reference counterpart: the reference's device layout is its bucket
+ * buffers (hmat_cuda.cc:127-257), which have no such export. */
+typedef struct bw_layout bw_layout;
+int bw_layout_build(const bw_flat_hmat* h, int precision, int adjoint, int64_t wi_bytes,
+                    bw_layout** out);
+int bw_layout_info(const bw_layout* lay, int64_t* info /* 16 */);
+int bw_layout_export(const bw_layout* lay, int pass, int64_t* wis, int32_t* gidx,
+                     int64_t* reds, double* data);
+void bw_layout_free(bw_layout* lay);
+const char* bw_layout_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

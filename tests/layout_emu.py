@@ -1,0 +1,104 @@
+"""CPU replay of the slab layout (test helper).
+
+Exports the layout through bw_layout_* (host-only C-ABI of libbwhip.so),
+checks every index each kernel would touch against its buffer, and replays
+k_gather / k_slab / k_reduce in numpy, so the layout is verified on the CPU
+before any kernel runs.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from bigwham_dev_amd import _native as N
+
+
+class LayoutDump:
+    def __init__(self, flat, precision="fp64", adjoint=False, wi_bytes=0):
+        lib = N.hip()
+        out = C.c_void_p()
+        prec = N.BW_PREC_FP64 if precision == "fp64" else N.BW_PREC_FP32
+        rc = lib.bw_layout_build(C.byref(flat), prec, int(adjoint), int(wi_bytes), C.byref(out))
+        if rc != 0:
+            raise N.BigWhamError(rc, lib.bw_layout_last_error().decode())
+        info = np.zeros(16, dtype=np.int64)
+        lib.bw_layout_info(out, info.ctypes.data_as(C.POINTER(C.c_int64)))
+        self.xt_t, self.xt_size, self.V = int(info[0]), int(info[1]), int(info[14])
+        self.passes = []
+        for p in range(2):
+            nwi, ngidx, nred, ndata, npart, nout = (int(v) for v in info[2 + 6 * p: 8 + 6 * p])
+            wis = np.zeros((max(nwi, 1), 8), dtype=np.int64)
+            gidx = np.zeros(max(ngidx, 1), dtype=np.int32)
+            reds = np.zeros((max(nred, 1), 4), dtype=np.int64)
+            data = np.zeros(max(ndata, 1))
+            lib.bw_layout_export(out, p + 1, wis.ctypes.data_as(C.POINTER(C.c_int64)),
+                                 gidx.ctypes.data_as(C.POINTER(C.c_int32)),
+                                 reds.ctypes.data_as(C.POINTER(C.c_int64)),
+                                 data.ctypes.data_as(C.POINTER(C.c_double)))
+            self.passes.append(dict(wis=wis[:nwi], gidx=gidx[:ngidx], reds=reds[:nred],
+                                    data=data[:ndata], npart=npart, nout=nout))
+        lib.bw_layout_free(out)
+        self.dof = flat.dof
+        self.n_in = flat.dof * (flat.n_row_pts if adjoint else flat.n_col_pts)
+        self.n_out = flat.dof * (flat.n_col_pts if adjoint else flat.n_row_pts)
+
+    def check_bounds(self):
+        """Every address k_slab / k_reduce would form lies inside its buffer."""
+        V = self.V
+        for p, P in enumerate(self.passes):
+            for (doff, gbeg, ng, R, orow, poff, _, _) in P["wis"]:
+                assert 1 <= R <= 64
+                assert doff >= 0 and doff + ng * R * V <= P["data"].size
+                assert gbeg >= 0 and gbeg + ng <= P["gidx"].size
+                if ng:
+                    z = P["gidx"][gbeg:gbeg + ng].astype(np.int64)
+                    assert z.min() >= 0 and z.max() + V <= self.xt_size
+                if poff >= 0:
+                    assert poff + R <= P["npart"]
+                elif p == 0:
+                    assert orow >= self.xt_t and orow + R <= self.xt_size - 64
+                else:
+                    assert orow >= 0 and orow + R <= self.n_out
+            for (orow, R, poff, npart) in P["reds"]:
+                assert poff + npart * R <= P["npart"]
+                if p == 0:
+                    assert orow >= self.xt_t and orow + R <= self.xt_size - 64
+                else:
+                    assert 0 <= orow and orow + R <= self.n_out
+
+    def replay(self, x: np.ndarray, perm_in=None, perm_out=None, dtype=np.float64):
+        """y = H x through the layout, with the kernels' arithmetic order."""
+        V, d = self.V, self.dof
+        xt = np.zeros(self.xt_size, dtype=dtype)
+        if perm_in is not None:
+            idx = (d * perm_in[:, None] + np.arange(d)).ravel()
+            xt[:self.n_in] = x[idx]
+        else:
+            xt[:self.n_in] = x
+        y = np.zeros(self.n_out)
+        for p, P in enumerate(self.passes):
+            part = np.zeros(P["npart"], dtype=dtype)
+            for (doff, gbeg, ng, R, orow, poff, _, _) in P["wis"]:
+                blk = P["data"][doff:doff + ng * R * V].reshape(ng, R, V).astype(dtype)
+                z = P["gidx"][gbeg:gbeg + ng].astype(np.int64)
+                zz = xt[z[:, None] + np.arange(V)[None, :]]  # (ng, V)
+                acc = np.einsum("grv,gv->r", blk, zz)
+                if poff >= 0:
+                    part[poff:poff + R] = acc
+                elif p == 0:
+                    xt[orow:orow + R] = acc
+                else:
+                    y[orow:orow + R] = acc
+            for (orow, R, poff, npart) in P["reds"]:
+                s = part[poff:poff + npart * R].reshape(npart, R).sum(0)
+                if p == 0:
+                    xt[orow:orow + R] = s
+                else:
+                    y[orow:orow + R] = s
+        if perm_out is not None:
+            idx = (d * perm_out[:, None] + np.arange(d)).ravel()
+            out = np.empty_like(y)
+            out[idx] = y
+            return out
+        return y

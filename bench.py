@@ -1,0 +1,225 @@
+"""H-matvec benchmark on one MI355X.
+
+A step is one H-matvec y = H x in the original DoF ordering
+(BigWhamIO::MatVec, src/io/bigwham_io.cc:919-941) over one synthetic input
+vector already resident in HBM.  The timed region is `--steps` FP64 matvecs
+bracketed by torch.cuda.synchronize(); `value` is matvecs per second.
+
+Alongside the headline line it reports:
+  roofline      the dominant kernel's algorithmic bytes / its average HIP-event
+                duration over the timed steps, against 8 TB/s;
+  fp32          the same for FP32 factors with FP32 accumulation;
+  gmres         time to ||b - H64 x|| / ||b|| <= 1e-8 for the FP64 path and the
+                mixed-precision path (--gmres);
+  cpu_baseline  the CPU oracle's OpenMP matvec (restatement of Hmat::matvec,
+                hmat.cc:472-537) on this host's cores, on the same H.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+CONFIGS = {
+    "c1": dict(kernel="2DS0-H", mesh=("line", 2000), leaf=32, eta=3.0, eps=1e-6,
+               workload="2DS0-H straight crack, 2000 segments (4000 DoF), leaf 32, eta 3, ACA eps 1e-6"),
+    "c2": dict(kernel="3DR0-H", mesh=("grid", 200), leaf=64, eta=3.0, eps=1e-6,
+               workload="3DR0-H square crack 200x200 rectangles (120000 DoF), leaf 64, eta 3, ACA eps 1e-6"),
+    "c3": dict(kernel="3DT0-H", mesh=("penny", 105000), leaf=64, eta=3.0, eps=1e-6,
+               workload="3DT0-H penny crack r=1, ~200k Delaunay triangles (~600k DoF), leaf 64, eta 3, ACA eps 1e-6"),
+    "c5": dict(kernel="3DR0-H-mode1", mesh=("grid", 500), leaf=64, eta=3.0, eps=1e-6,
+               workload="3DR0-H-mode1 500x500 rectangles (250000 DoF), leaf 64, eta 3, ACA eps 1e-6"),
+}
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def make_mesh(kind, n):
+    from bigwham_dev_amd import meshes
+    if kind == "line":
+        return meshes.segment_line(n)
+    if kind == "grid":
+        return meshes.rect_grid(n)
+    if kind == "penny":
+        return meshes.penny_delaunay(n, seed=0)
+    raise ValueError(kind)
+
+
+def split_entries(host):
+    """Entries of the F blocks, the A factors and the B factors; total rank T."""
+    fp, lp = host.fr_pattern(), host.lr_pattern()
+    d = host.dof
+    e_fr = int(np.sum(d * (fp[3] - fp[1]) * d * (fp[4] - fp[2])))
+    e_a = int(np.sum(d * (lp[3] - lp[1]) * lp[5]))
+    e_b = int(np.sum(lp[5] * d * (lp[4] - lp[2])))
+    return e_fr, e_a, e_b, int(np.sum(lp[5]))
+
+
+def cpu_baseline(host, x, budget_s=15.0):
+    """Time the oracle's OpenMP matvec (1 warm-up + up to 10 timed calls,
+    mat_vec.cc:158-168 protocol) within a time budget."""
+    import oracle as O
+    flat = host.flat()
+    threads = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16")))
+    t0 = time.perf_counter()
+    y = O.matvec_original(flat, x, threads)
+    t_warm = time.perf_counter() - t0
+    reps = int(max(1, min(10, budget_s // max(t_warm, 1e-6))))
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        O.matvec_original(flat, x, threads)
+        ts.append(time.perf_counter() - t0)
+    med = float(np.median(ts))
+    return y, dict(value=1.0 / med, unit="H-matvec/s", cores=threads, kind="port",
+                   sample=f"{reps} timed calls (+1 warm-up) of the full matvec, median "
+                          f"{med * 1e3:.2f} ms, OpenMP restatement of Hmat::matvecOriginal")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--config", default=os.environ.get("BW_BENCH_CONFIG", "c2"))
+    ap.add_argument("--gmres", action="store_true", help="also time GMRES to 1e-8")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--wi-bytes", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only; multi-GPU sharding is a design "
+              "note (block-row ownership + RCCL all-gather of x), not a measured path.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from bigwham_dev_amd import HipHmat, HostHmat, set_tuning
+    from bigwham_dev_amd import solvers
+
+    cfg = CONFIGS[args.config]
+    coor, conn = make_mesh(*cfg["mesh"])
+    t0 = time.perf_counter()
+    host = HostHmat.from_mesh(cfg["kernel"], coor, conn, [1.0, 0.25])
+    host.build(cfg["leaf"], cfg["eta"], cfg["eps"])
+    t_build = time.perf_counter() - t0
+    n = host.size
+    E = host.nb_entries()
+    e_fr, e_a, e_b, T = split_entries(host)
+    log(f"config {args.config}: N={n} E={E} (FR {e_fr}, A {e_a}, B {e_b}) build {t_build:.1f}s")
+
+    if args.wi_bytes:
+        set_tuning(args.wi_bytes, args.wi_bytes)
+    t0 = time.perf_counter()
+    dev = HipHmat(host, precisions=("fp64", "fp32"), device=0)
+    torch.cuda.synchronize()
+    t_upload = time.perf_counter() - t0
+    log(f"layout + upload {t_upload:.1f}s, device bytes {dev.device_bytes / 1e9:.2f} GB")
+
+    rng = np.random.default_rng(1234)
+    x_np = rng.uniform(-1.0, 1.0, n)
+    x = torch.from_numpy(x_np).to("cuda:0")
+    y = torch.empty(n, dtype=torch.float64, device="cuda:0")
+
+    def run(precision, steps, warmup):
+        for _ in range(warmup):
+            dev.matvec(x, original=True, precision=precision, out=y)
+        dev.set_timing(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            dev.matvec(x, original=True, precision=precision, out=y)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        stats = dev.timing_stats()
+        dev.set_timing(False)
+        return dt, stats
+
+    dt64, st64 = run("fp64", args.steps, args.warmup)
+    y64 = y.cpu().numpy()
+    dt32, st32 = run("fp32", args.steps, args.warmup)
+    y32 = y.cpu().numpy()
+    log(f"fp64 {dt64 / args.steps * 1e3:.3f} ms/matvec, fp32 {dt32 / args.steps * 1e3:.3f} ms/matvec")
+    log(f"fp64 kernels {st64}")
+    log(f"fp32 kernels {st32}")
+
+    def roofline(stats, sm, sv):
+        # algorithmic bytes per launch of each slab kernel
+        b1 = sm * e_b + sv * (n + T)
+        b2 = sm * (e_fr + e_a) + sv * (n + T + n)
+        k1, k2 = stats["p1_slab"], stats["p2_slab"]
+        dom, b, ms = ("k_slab pass 2", b2, k2) if k2 >= k1 else ("k_slab pass 1", b1, k1)
+        ach = b / (ms * 1e-3) / 1e9
+        whole = (sm * E + sv * 4 * n + 2 * 8 * n / host.dof) / (stats["total"] * 1e-3) / 1e9
+        return dict(bound="hbm", kernel=dom, achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(ach / HBM_PEAK_GBS, 4), traffic=None,
+                    avg_kernel_ms=round(ms, 4), alg_bytes=int(b),
+                    whole_matvec_GBs=round(whole, 1),
+                    whole_matvec_frac=round(whole / HBM_PEAK_GBS, 4))
+
+    rf64 = roofline(st64, 8, 8)
+    rf32 = roofline(st32, 4, 4)
+
+    out = {
+        "metric": "H-matvec/s",
+        "value": round(args.steps / dt64, 3),
+        "unit": "matvec/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt64 / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": cfg["workload"], "n_dof": n, "nb_entries": E,
+                   "compression_ratio": round(E / float(n) / n, 6),
+                   "ordering": "original", "inputs": "x ~ U[-1,1] seed 1234, resident in HBM"},
+        "roofline": rf64,
+        "fp32": {"value": round(args.steps / dt32, 3), "unit": "matvec/s",
+                 "ms_per_step": round(dt32 / args.steps * 1e3, 4), "roofline": rf32},
+        "setup_s": {"host_build": round(t_build, 2), "layout_upload": round(t_upload, 2)},
+    }
+
+    if not args.no_cpu:
+        y_ref, cb = cpu_baseline(host, x_np)
+        out["cpu_baseline"] = cb
+        out["parity"] = {
+            "fp64_rel_err_vs_oracle": float(np.linalg.norm(y64 - y_ref) / np.linalg.norm(y_ref)),
+            "fp32_rel_err_vs_oracle": float(np.linalg.norm(y32 - y_ref) / np.linalg.norm(y_ref)),
+        }
+        out["vs_cpu"] = round(out["value"] / cb["value"], 2)
+
+    if args.gmres:
+        b = torch.zeros(n, dtype=torch.float64, device="cuda:0")
+        b[host.dof - 1::host.dof] = 1.0  # unit normal pressure, local frame (normal last)
+        x64, info64 = solvers.solve_fp64(dev, b, tol=1e-8)
+        xm, infom = solvers.solve_mixed(dev, b, tol=1e-8)
+        out["gmres"] = {
+            "tol": 1e-8, "restart": 30, "precond": "jacobi",
+            "fp64": {"seconds": round(info64.seconds, 4), "iterations": info64.iterations,
+                     "rel_residual_fp64": info64.rel_residual},
+            "mixed": {"seconds": round(infom.seconds, 4), "iterations": infom.iterations,
+                      "fp32_matvecs": infom.matvecs_fp32, "fp64_matvecs": infom.matvecs_fp64,
+                      "rel_residual_fp64": infom.rel_residual},
+            "speedup_mixed_vs_fp64": round(info64.seconds / infom.seconds, 3),
+            "solution_rel_diff": float(torch.linalg.vector_norm(xm - x64) /
+                                       torch.linalg.vector_norm(x64)),
+        }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -342,9 +342,16 @@ struct bw_hip_handle {
   std::vector<double> diag_perm, diag_orig;
   std::vector<void*> allocs;
   int64_t device_bytes = 0;
+  // Kernel timing: two alternating sets of events between the kernels of a
+  // matvec; a set is read (and accumulated) only when reused two calls later,
+  // so timing never stalls the launch pipeline.
   bool timing = false;
-  hipEvent_t ev[4] = {};
-  float last_ms[4] = {0, 0, 0, 0};
+  hipEvent_t evs[2][6] = {};
+  hipEvent_t* ev = evs[0];
+  bool pending[2] = {false, false};
+  int cur = 0;
+  double sum_ms[6] = {0, 0, 0, 0, 0, 0};
+  int64_t n_timed = 0;
   std::mutex mu;
 };
 
@@ -423,23 +430,40 @@ int launch_matvec(bw_hip_handle* hd, DevOp& op, const double* xd, double* yd, in
     hipLaunchKernelGGL((k_slab<T, T, 8>), dim3(blocks), dim3(256), 0, st, op.p1.wis, op.p1.nwi,
                        op.p1.gidx, static_cast<const T*>(op.p1.data), xt, xt,
                        static_cast<T*>(op.p1.part), nullptr, dof);
-    if (op.p1.nred > 0)
-      hipLaunchKernelGGL((k_reduce<T, T>), dim3(op.p1.nred), dim3(64), 0, st, op.p1.reds,
-                         op.p1.nred, static_cast<const T*>(op.p1.part), xt, nullptr, dof);
   }
   if (hd->timing) hipEventRecord(hd->ev[2], st);
+  if (op.p1.nred > 0)
+    hipLaunchKernelGGL((k_reduce<T, T>), dim3(op.p1.nred), dim3(64), 0, st, op.p1.reds,
+                       op.p1.nred, static_cast<const T*>(op.p1.part), xt, nullptr, dof);
+  if (hd->timing) hipEventRecord(hd->ev[3], st);
   if (op.p2.nwi > 0) {
     const int blocks = (op.p2.nwi + 3) / 4;
     hipLaunchKernelGGL((k_slab<T, double, 8>), dim3(blocks), dim3(256), 0, st, op.p2.wis,
                        op.p2.nwi, op.p2.gidx, static_cast<const T*>(op.p2.data), xt, yd,
                        static_cast<T*>(op.p2.part), orig ? op.perm_out : nullptr, dof);
-    if (op.p2.nred > 0)
-      hipLaunchKernelGGL((k_reduce<T, double>), dim3(op.p2.nred), dim3(64), 0, st, op.p2.reds,
-                         op.p2.nred, static_cast<const T*>(op.p2.part), yd,
-                         orig ? op.perm_out : nullptr, dof);
   }
-  if (hd->timing) hipEventRecord(hd->ev[3], st);
+  if (hd->timing) hipEventRecord(hd->ev[4], st);
+  if (op.p2.nred > 0)
+    hipLaunchKernelGGL((k_reduce<T, double>), dim3(op.p2.nred), dim3(64), 0, st, op.p2.reds,
+                       op.p2.nred, static_cast<const T*>(op.p2.part), yd,
+                       orig ? op.perm_out : nullptr, dof);
+  if (hd->timing) hipEventRecord(hd->ev[5], st);
   HIPCHK(hipGetLastError());
+  return BW_OK;
+}
+
+int accumulate_timing(bw_hip_handle* hd, int set) {
+  hipEvent_t* e = hd->evs[set];
+  HIPCHK(hipEventSynchronize(e[5]));
+  float ms;
+  for (int i = 0; i < 5; ++i) {
+    HIPCHK(hipEventElapsedTime(&ms, e[i], e[i + 1]));
+    hd->sum_ms[i] += ms;
+  }
+  HIPCHK(hipEventElapsedTime(&ms, e[0], e[5]));
+  hd->sum_ms[5] += ms;
+  hd->n_timed += 1;
+  hd->pending[set] = false;
   return BW_OK;
 }
 
@@ -463,19 +487,24 @@ int do_matvec(bw_hip_handle* hd, bool adjoint, const double* x, double* y, int o
     xd = hd->x_stage;
     yd = hd->y_stage;
   }
+  if (hd->timing) {
+    const int set = hd->cur;
+    if (hd->pending[set]) {
+      int rc0 = accumulate_timing(hd, set);
+      if (rc0) return rc0;
+    }
+    hd->ev = hd->evs[set];
+  }
   int rc = precision == BW_PREC_FP64 ? launch_matvec<double>(hd, op, xd, yd, ordering, st)
                                      : launch_matvec<float>(hd, op, xd, yd, ordering, st);
   if (rc) return rc;
+  if (hd->timing) {
+    hd->pending[hd->cur] = true;
+    hd->cur ^= 1;
+  }
   if (!on_device) {
     HIPCHK(hipMemcpyAsync(y, hd->y_stage, op.n_out * sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
-  }
-  if (hd->timing) {
-    HIPCHK(hipEventSynchronize(hd->ev[3]));
-    hipEventElapsedTime(&hd->last_ms[0], hd->ev[0], hd->ev[1]);
-    hipEventElapsedTime(&hd->last_ms[1], hd->ev[1], hd->ev[2]);
-    hipEventElapsedTime(&hd->last_ms[2], hd->ev[2], hd->ev[3]);
-    hipEventElapsedTime(&hd->last_ms[3], hd->ev[0], hd->ev[3]);
   }
   return BW_OK;
 }
@@ -498,8 +527,9 @@ void bw_hip_destroy(bw_hip_handle* hd) {
   hipSetDevice(hd->device);
   hipDeviceSynchronize();
   for (void* p : hd->allocs) hipFree(p);
-  for (auto& e : hd->ev)
-    if (e) hipEventDestroy(e);
+  for (auto& set : hd->evs)
+    for (auto& e : set)
+      if (e) hipEventDestroy(e);
   delete hd;
 }
 
@@ -578,8 +608,9 @@ int bw_hip_create(const bw_flat_hmat* h, int precision_mask, int flags, int devi
   }
   hd->nb_entries = E;
   compute_diagonal(hd, h);
-  for (auto& e : hd->ev)
-    if (hipEventCreate(&e) != hipSuccess) return cleanup(fail(BW_EHIP, "create: hipEventCreate"));
+  for (auto& set : hd->evs)
+    for (auto& e : set)
+      if (hipEventCreate(&e) != hipSuccess) return cleanup(fail(BW_EHIP, "create: hipEventCreate"));
   *out = hd;
   return BW_OK;
 }
@@ -630,13 +661,27 @@ int bw_hip_sizes(bw_hip_handle* hd, int64_t* n_rows, int64_t* n_cols, int64_t* n
 
 int bw_hip_set_timing(bw_hip_handle* hd, int enable) {
   if (!hd) return fail(BW_EINVAL, "set_timing: null handle");
+  for (int s = 0; s < 2; ++s)
+    if (hd->pending[s]) HIPCHK(hipEventSynchronize(hd->evs[s][5]));
+  hd->pending[0] = hd->pending[1] = false;
+  hd->cur = 0;
+  for (double& v : hd->sum_ms) v = 0.0;
+  hd->n_timed = 0;
   hd->timing = enable != 0;
   return BW_OK;
 }
 
-int bw_hip_last_timing(bw_hip_handle* hd, float* ms4) {
-  if (!hd || !ms4) return fail(BW_EINVAL, "last_timing: null argument");
-  std::memcpy(ms4, hd->last_ms, sizeof(hd->last_ms));
+int bw_hip_timing_stats(bw_hip_handle* hd, double* avg_ms6, int64_t* count) {
+  if (!hd || !avg_ms6) return fail(BW_EINVAL, "timing_stats: null argument");
+  for (int s = 0; s < 2; ++s) {
+    const int set = (hd->cur + s) & 1;  // oldest first
+    if (hd->pending[set]) {
+      int rc = accumulate_timing(hd, set);
+      if (rc) return rc;
+    }
+  }
+  for (int i = 0; i < 6; ++i) avg_ms6[i] = hd->n_timed ? hd->sum_ms[i] / hd->n_timed : 0.0;
+  if (count) *count = hd->n_timed;
   return BW_OK;
 }
 

@@ -266,10 +266,16 @@ class HipHmat:
     def set_timing(self, enable: bool = True) -> None:
         N.check_hip(N.hip().bw_hip_set_timing(self._h, int(enable)))
 
-    def last_timing(self):
-        ms = (C.c_float * 4)()
-        N.check_hip(N.hip().bw_hip_last_timing(self._h, ms))
-        return list(ms)
+    def timing_stats(self):
+        """Average ms per matvec since set_timing(): dict of the five kernels
+        (gather, p1_slab, p1_reduce, p2_slab, p2_reduce), 'total' and 'count'."""
+        avg = np.zeros(6)
+        n = C.c_int64()
+        N.check_hip(N.hip().bw_hip_timing_stats(self._h, N.dptr(avg), C.byref(n)))
+        keys = ["gather", "p1_slab", "p1_reduce", "p2_slab", "p2_reduce", "total"]
+        out = dict(zip(keys, avg.tolist()))
+        out["count"] = n.value
+        return out
 
 
 def set_tuning(wi_bytes_fp64: int = 0, wi_bytes_fp32: int = 0) -> None:

@@ -83,11 +83,13 @@ int bw_hip_sizes(bw_hip_handle* hd, int64_t* n_rows, int64_t* n_cols,
 /* Tuning knobs for the next create (work-item target bytes; 0 = default). */
 int bw_hip_set_tuning(int64_t wi_bytes_fp64, int64_t wi_bytes_fp32);
 
-/* Per-kernel timing of the last matvec issued with timing enabled:
- * enable != 0 records HIP events around each pass on the call's stream;
- * ms[0..3] = gather, pass 1, pass 2 (incl. reduce), total. */
+/* Per-kernel timing.  With timing enabled every matvec records HIP events
+ * between its kernels on the call's stream; bw_hip_timing_stats returns the
+ * average milliseconds over the calls since bw_hip_set_timing (which resets
+ * the averages): avg[0..5] = gather, pass-1 slab, pass-1 reduce, pass-2 slab,
+ * pass-2 reduce, whole matvec; *count = number of calls averaged. */
 int bw_hip_set_timing(bw_hip_handle* hd, int enable);
-int bw_hip_last_timing(bw_hip_handle* hd, float* ms4);
+int bw_hip_timing_stats(bw_hip_handle* hd, double* avg_ms6, int64_t* count);
 
 void bw_hip_destroy(bw_hip_handle* hd);
 const char* bw_hip_last_error(void);

@@ -10,7 +10,7 @@ import pytest
 
 import oracle as O
 from bigwham_dev_amd import HipHmat, HostHmat
-from meshes import rect_grid, segment_line
+from bigwham_dev_amd.meshes import rect_grid, segment_line
 
 pytestmark = pytest.mark.gpu
 

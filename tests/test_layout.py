@@ -6,7 +6,7 @@ import pytest
 import oracle as O
 from bigwham_dev_amd import HostHmat
 from layout_emu import LayoutDump
-from meshes import rect_grid, segment_line
+from bigwham_dev_amd.meshes import rect_grid, segment_line
 
 
 def _cases():

@@ -27,6 +27,7 @@ import numpy as np
 
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
+sys.path.insert(0, str(ROOT / "oracle"))  # cpu_baseline leg only
 
 HBM_PEAK_GBS = 8000.0
 
@@ -205,6 +206,7 @@ def main():
     if args.gmres:
         b = torch.zeros(n, dtype=torch.float64, device="cuda:0")
         b[host.dof - 1::host.dof] = 1.0  # unit normal pressure, local frame (normal last)
+        solvers.warm_up(dev, b)
         x64, info64 = solvers.solve_fp64(dev, b, tol=1e-8)
         xm, infom = solvers.solve_mixed(dev, b, tol=1e-8)
         out["gmres"] = {

@@ -90,8 +90,9 @@ __device__ __forceinline__ int64_t out_index(int64_t idx, const int64_t* perm, i
 
 // One wavefront per work item, four per workgroup.
 // T: factor / accumulation type; OutT: output type (T for pass 1, double
-// for pass 2); U: unroll depth (independent 16-byte loads in flight per lane).
-template <typename T, typename OutT, int U>
+// for pass 2); U: unroll depth (independent 16-byte loads in flight per lane);
+// PASS only names the two launches apart in profiles.
+template <typename T, typename OutT, int U, int PASS>
 __global__ __launch_bounds__(256) void k_slab(const bwl::WorkItem* __restrict__ wis, int nwi,
                                               const int32_t* __restrict__ gidx,
                                               const T* __restrict__ data,
@@ -427,7 +428,7 @@ int launch_matvec(bw_hip_handle* hd, DevOp& op, const double* xd, double* yd, in
   if (hd->timing) hipEventRecord(hd->ev[1], st);
   if (op.p1.nwi > 0) {
     const int blocks = (op.p1.nwi + 3) / 4;
-    hipLaunchKernelGGL((k_slab<T, T, 8>), dim3(blocks), dim3(256), 0, st, op.p1.wis, op.p1.nwi,
+    hipLaunchKernelGGL((k_slab<T, T, 8, 1>), dim3(blocks), dim3(256), 0, st, op.p1.wis, op.p1.nwi,
                        op.p1.gidx, static_cast<const T*>(op.p1.data), xt, xt,
                        static_cast<T*>(op.p1.part), nullptr, dof);
   }
@@ -438,7 +439,7 @@ int launch_matvec(bw_hip_handle* hd, DevOp& op, const double* xd, double* yd, in
   if (hd->timing) hipEventRecord(hd->ev[3], st);
   if (op.p2.nwi > 0) {
     const int blocks = (op.p2.nwi + 3) / 4;
-    hipLaunchKernelGGL((k_slab<T, double, 8>), dim3(blocks), dim3(256), 0, st, op.p2.wis,
+    hipLaunchKernelGGL((k_slab<T, double, 8, 2>), dim3(blocks), dim3(256), 0, st, op.p2.wis,
                        op.p2.nwi, op.p2.gidx, static_cast<const T*>(op.p2.data), xt, yd,
                        static_cast<T*>(op.p2.part), orig ? op.perm_out : nullptr, dof);
   }

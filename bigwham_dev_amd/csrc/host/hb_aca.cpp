@@ -12,8 +12,9 @@
 //     (adaptiveCrossApproximation.h:127-212)
 //   * next row pivot = the unused row block of the new A column block with
 //     the largest smallest singular value (routines.h:144-184)
-// Singular values of the p x p blocks come from one-sided Jacobi rotations
-// (p = 3) or closed forms (p = 1, 2) instead of LAPACK; the LU inverse is
+// Smallest singular values of the p x p blocks come from closed forms instead
+// of LAPACK (p = 3: |det| over the product of the two largest singular
+// values, those from the Gram matrix eigenvalues); the LU inverse is
 // Gaussian elimination with partial pivoting.  The reference recomputes the
 // residual row after extending A; that recomputation uses only the first
 // `rank` column blocks of A and gives the same values, so it is skipped.
@@ -36,42 +37,45 @@ double smallest_singular_value(int p, const double* M) {
     if (smax == 0.0) return 0.0;
     return det / smax;
   }
-  // one-sided Jacobi (Hestenes) on the 3 columns
-  double U[9];
-  std::memcpy(U, M, sizeof(U));
-  for (int sweep = 0; sweep < 30; ++sweep) {
-    double off = 0.0;
-    for (int i = 0; i < 2; ++i) {
-      for (int j = i + 1; j < 3; ++j) {
-        double alpha = 0, beta = 0, gamma = 0;
-        for (int k = 0; k < 3; ++k) {
-          alpha += U[i * 3 + k] * U[i * 3 + k];
-          beta += U[j * 3 + k] * U[j * 3 + k];
-          gamma += U[i * 3 + k] * U[j * 3 + k];
-        }
-        if (gamma == 0.0) continue;
-        off = std::max(off, std::fabs(gamma) / std::sqrt(alpha * beta));
-        const double zeta = (beta - alpha) / (2.0 * gamma);
-        const double t = (zeta >= 0 ? 1.0 : -1.0) /
-                         (std::fabs(zeta) + std::sqrt(1.0 + zeta * zeta));
-        const double cs = 1.0 / std::sqrt(1.0 + t * t);
-        const double sn = cs * t;
-        for (int k = 0; k < 3; ++k) {
-          const double ui = U[i * 3 + k], uj = U[j * 3 + k];
-          U[i * 3 + k] = cs * ui - sn * uj;
-          U[j * 3 + k] = sn * ui + cs * uj;
-        }
-      }
-    }
-    if (off < 1e-15) break;
+  // sigma_min = |det M| / (sigma_max sigma_mid), with sigma_max^2 and
+  // sigma_mid^2 the two largest eigenvalues of the Gram matrix M^T M from the
+  // trigonometric closed form for symmetric 3 x 3 matrices.  The two large
+  // eigenvalues are well conditioned and det M is computed directly, so the
+  // quotient keeps its relative accuracy even for nearly singular blocks.
+  const double* c0 = M;
+  const double* c1 = M + 3;
+  const double* c2 = M + 6;
+  auto dot3 = [](const double* u, const double* v) { return u[0] * v[0] + u[1] * v[1] + u[2] * v[2]; };
+  const double g00 = dot3(c0, c0), g11 = dot3(c1, c1), g22 = dot3(c2, c2);
+  const double g01 = dot3(c0, c1), g02 = dot3(c0, c2), g12 = dot3(c1, c2);
+  const double det = std::fabs(c0[0] * (c1[1] * c2[2] - c1[2] * c2[1]) -
+                               c1[0] * (c0[1] * c2[2] - c0[2] * c2[1]) +
+                               c2[0] * (c0[1] * c1[2] - c0[2] * c1[1]));
+  if (det == 0.0) return 0.0;
+  const double off = g01 * g01 + g02 * g02 + g12 * g12;
+  const double q = (g00 + g11 + g22) / 3.0;
+  double lmax, lmid;
+  if (off == 0.0) {
+    double e[3] = {g00, g11, g22};
+    std::sort(e, e + 3);
+    lmax = e[2];
+    lmid = e[1];
+  } else {
+    const double a = g00 - q, b = g11 - q, c = g22 - q;
+    const double pp = std::sqrt((a * a + b * b + c * c + 2.0 * off) / 6.0);
+    const double ia = a / pp, ib = b / pp, ic = c / pp;
+    const double f01 = g01 / pp, f02 = g02 / pp, f12 = g12 / pp;
+    double r = (ia * (ib * ic - f12 * f12) - f01 * (f01 * ic - f12 * f02) +
+                f02 * (f01 * f12 - ib * f02)) / 2.0;
+    r = std::min(1.0, std::max(-1.0, r));
+    const double phi = std::acos(r) / 3.0;
+    lmax = q + 2.0 * pp * std::cos(phi);
+    const double lmin = q + 2.0 * pp * std::cos(phi + 2.0943951023931953);
+    lmid = 3.0 * q - lmax - lmin;
   }
-  double smin = INFINITY;
-  for (int i = 0; i < 3; ++i) {
-    const double s = std::sqrt(U[i * 3] * U[i * 3] + U[i * 3 + 1] * U[i * 3 + 1] +
-                               U[i * 3 + 2] * U[i * 3 + 2]);
-    smin = std::min(smin, s);
-  }
-  return smin;
+  const double prod = lmax * lmid;
+  if (!(prod > 0.0)) return 0.0;
+  return det / std::sqrt(prod);
 }
 
 bool invert_small(int p, const double* M, double* Minv) {
@@ -129,31 +133,33 @@ LowRankBlock aca(const Generator& M, int64_t r0b, int64_t r0e, int64_t r1b,
 
   auto residual_row = [&](int64_t ii0) {
     M.set(ii0, r1b, 1, n1, row.data(), p);
-    if (rank >= 1) {
-      // row(b, j) -= sum_q A(ii0*p + b, q) * B(q, j)
-      const int64_t ar = (ii0 - r0b) * p;
-      for (int64_t j = 0; j < m1; ++j) {
-        for (int b = 0; b < p; ++b) {
-          double s = 0.0;
-          for (int64_t rr = 0; rr < rank; ++rr)
-            for (int q = 0; q < p; ++q)
-              s += Acols[(ar + b) + (rr * p + q) * m0] * Brows[rr][q + j * p];
-          row[b + j * p] -= s;
+    // row(b, j) -= sum_{rr, q} A(ii0*p + b, rr*p + q) * B(rr*p + q, j), one
+    // rank term at a time so that each B row block is streamed contiguously
+    const int64_t ar = (ii0 - r0b) * p;
+    for (int64_t rr = 0; rr < rank; ++rr) {
+      const double* Br = Brows[rr].data();
+      for (int q = 0; q < p; ++q) {
+        double a[3];
+        for (int b = 0; b < p; ++b) a[b] = Acols[(ar + b) + (rr * p + q) * m0];
+        for (int64_t j = 0; j < m1; ++j) {
+          const double bq = Br[q + j * p];
+          for (int b = 0; b < p; ++b) row[b + j * p] -= a[b] * bq;
         }
       }
     }
   };
   auto residual_col = [&](int64_t ii1) {
     M.set(r0b, ii1, n0, 1, col.data(), m0);
-    if (rank >= 1) {
-      const int64_t bc = (ii1 - r1b) * p;
-      for (int q1 = 0; q1 < p; ++q1) {
-        for (int64_t i = 0; i < m0; ++i) {
-          double s = 0.0;
-          for (int64_t rr = 0; rr < rank; ++rr)
-            for (int q = 0; q < p; ++q)
-              s += Acols[i + (rr * p + q) * m0] * Brows[rr][q + (bc + q1) * p];
-          col[i + q1 * m0] -= s;
+    // col(:, q1) -= sum_{rr, q} A(:, rr*p + q) * B(rr*p + q, ii1*p + q1)
+    const int64_t bc = (ii1 - r1b) * p;
+    for (int64_t rr = 0; rr < rank; ++rr) {
+      const double* Br = Brows[rr].data();
+      for (int q = 0; q < p; ++q) {
+        const double* Ac = &Acols[static_cast<size_t>(rr * p + q) * m0];
+        for (int q1 = 0; q1 < p; ++q1) {
+          const double bv = Br[q + (bc + q1) * p];
+          double* cc = &col[static_cast<size_t>(q1) * m0];
+          for (int64_t i = 0; i < m0; ++i) cc[i] -= Ac[i] * bv;
         }
       }
     }

@@ -12,6 +12,8 @@
 // kernel output lands at M(j0*dof + i, j1*dof + j) = st[j*dof + i].
 #include <algorithm>
 #include <atomic>
+#include <cstdio>
+#include <cstdlib>
 #include <chrono>
 #include <cstring>
 #include <mutex>
@@ -58,6 +60,14 @@ void parallel_for(int nthreads, int64_t n, int64_t chunk, F&& f) {
   for (int t = 1; t < nt; ++t) ths.emplace_back(worker);
   worker();
   for (auto& t : ths) t.join();
+}
+
+bool verbose() {
+  static const bool v = [] {
+    const char* e = std::getenv("BIGWHAM_VERBOSE");
+    return e && e[0] && e[0] != '0';
+  }();
+  return v;
 }
 
 double now_s() {
@@ -194,6 +204,7 @@ int bwh_create(const char* kernel, const double* coor, int64_t n_nodes,
   h->colloc.resize(n_elts * h->dim);
   for (int64_t e = 0; e < n_elts; ++e) {
     bwh::Element& el = h->elts[e];
+    el.id = e;
     if (h->dim == 2) {
       double x0[2] = {coor[h->conn[2 * e] * 2], coor[h->conn[2 * e] * 2 + 1]};
       double x1[2] = {coor[h->conn[2 * e + 1] * 2], coor[h->conn[2 * e + 1] * 2 + 1]};
@@ -206,6 +217,7 @@ int bwh_create(const char* kernel, const double* coor, int64_t n_nodes,
     }
     for (int j = 0; j < h->dim; ++j) h->colloc[e * h->dim + j] = el.colloc[j];
   }
+  h->kernel->prepare(h->elts);
   *out = h.release();
   return BW_OK;
 }
@@ -283,6 +295,10 @@ int bwh_build_hmat(bwh_hmat* h, int max_leaf_size, double eta, double eps_aca) {
   });
   h->fr_ptr.resize(nfr);
   for (int64_t k = 0; k < nfr; ++k) h->fr_ptr[k] = h->fr_data.data() + off[k];
+  const double t_fr = now_s() - t0;
+  if (verbose())
+    std::fprintf(stderr, "[bwh] full-rank blocks: %lld blocks, %lld entries, %.2f s\n",
+                 static_cast<long long>(nfr), static_cast<long long>(off[nfr]), t_fr);
   // LR: ACA per block
   h->lr_blocks.assign(nlr, bwh::LowRankBlock{});
   const int fixed = h->fixed_rank;
@@ -299,6 +315,12 @@ int bwh_build_hmat(bwh_hmat* h, int max_leaf_size, double eta, double eps_aca) {
   }
   h->built = true;
   h->hmat_time = now_s() - t0;
+  if (verbose()) {
+    int64_t elr = 0;
+    for (const auto& b : h->lr_blocks) elr += static_cast<int64_t>(b.A.size() + b.B.size());
+    std::fprintf(stderr, "[bwh] low-rank blocks: %lld blocks, %lld entries, %.2f s\n",
+                 static_cast<long long>(nlr), static_cast<long long>(elr), h->hmat_time - t_fr);
+  }
   return BW_OK;
 }
 

@@ -69,6 +69,31 @@ void exchange_sort_rows(std::vector<double>& pts, int dim,
   if (n <= 1) return;
   std::vector<double> key(n);
   for (int64_t i = 0; i < n; ++i) key[i] = pts[(b + i) * dim + d];
+  // With pairwise distinct keys the ascending arrangement is unique, so an
+  // O(n log n) sort yields exactly the exchange sort's result.  (On scattered
+  // coordinates the exchange sort performs ~n^2/4 swaps.)
+  {
+    std::vector<int64_t> idx(n);
+    for (int64_t i = 0; i < n; ++i) idx[i] = i;
+    std::sort(idx.begin(), idx.end(), [&](int64_t a, int64_t c) { return key[a] < key[c]; });
+    bool distinct = true;
+    for (int64_t i = 1; i < n && distinct; ++i)
+      if (!(key[idx[i - 1]] < key[idx[i]])) distinct = false;
+    if (distinct) {
+      std::vector<double> rows(static_cast<size_t>(n) * dim);
+      std::vector<int64_t> pm(n);
+      for (int64_t i = 0; i < n; ++i) {
+        for (int q = 0; q < dim; ++q) rows[i * dim + q] = pts[(b + idx[i]) * dim + q];
+        pm[i] = perm[b + idx[i]];
+      }
+      std::copy(rows.begin(), rows.end(), pts.begin() + b * dim);
+      std::copy(pm.begin(), pm.end(), perm.begin() + b);
+      return;
+    }
+  }
+  // Ties: replay the exchange sort pass by pass.  Pass i swaps position i
+  // with each later position holding a value strictly below the value
+  // currently at i; a min segment tree finds those positions.
   MinTree tree(key);
   auto swap_rows = [&](int64_t i, int64_t k) {
     for (int q = 0; q < dim; ++q) std::swap(pts[(b + i) * dim + q], pts[(b + k) * dim + q]);

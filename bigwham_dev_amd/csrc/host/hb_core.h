@@ -16,6 +16,7 @@
 namespace bwh {
 
 struct Element {
+  int64_t id = -1;  // index in the mesh
   int dim = 3;
   int nv = 0;
   double v[4][3] = {};
@@ -54,6 +55,9 @@ class Kernel {
   virtual int dof() const = 0;
   virtual void influence(const Element& src, const Element& rec,
                          double* out) const = 0;
+  // Optional per-mesh precomputation of source-only quantities (indexed by
+  // Element::id); called once after the mesh is built.
+  virtual void prepare(const std::vector<Element>& elts) { (void)elts; }
 };
 
 std::unique_ptr<Kernel> make_kernel(const std::string& name, const double* props,

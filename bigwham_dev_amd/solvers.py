@@ -113,6 +113,15 @@ def jacobi(diag: torch.Tensor):
     return lambda v: inv * v
 
 
+def warm_up(hmat, b: torch.Tensor, restart: int = 30) -> None:
+    """One short restart cycle per precision, so that library initialisation
+    (rocBLAS handles for the Gram-Schmidt products) is not timed."""
+    for prec in hmat.precisions:
+        gmres(lambda v: hmat.matvec(v, original=True, precision=prec), b, None,
+              tol=0.0, restart=restart, maxiter=2)
+    torch.cuda.synchronize()
+
+
 def solve_fp64(hmat, b: torch.Tensor, tol: float = 1e-8, restart: int = 30,
                maxiter: int = 2000, precondition: bool = True) -> tuple[torch.Tensor, SolveInfo]:
     """Plain FP64 GMRES on the FP64 H-matvec (the baseline path)."""

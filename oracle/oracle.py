@@ -21,7 +21,9 @@ def lib() -> C.CDLL:
     if _lib is None:
         path = HERE / "lib" / "liboracle.so"
         if not path.exists():
-            from . import build_oracle  # type: ignore
+            import sys
+            sys.path.insert(0, str(HERE))
+            import build_oracle
             build_oracle.build_oracle()
         _lib = C.CDLL(str(path))
         vp = C.c_void_p

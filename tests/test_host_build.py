@@ -1,0 +1,296 @@
+"""Host construction: kernels, cluster tree, pattern, ACA (CPU only).
+
+Pins:
+  * kernel known answers of the reference's own tests:
+      2DS0 self effect 0.34979115367667662 / 0.34979125861394933
+        (src/elasticity/_test/elastic_square_matrix_generator_test.cc:69-99)
+      3DR0 and 3DR0-mode1 self effect 0.391078
+        (fullspace_iso_3d_rectangle/_test/rectangle0_kernels_test.cc:20-41,
+         rectangle0_mode1_kernels_test.cc:20-37)
+      3DT0 self effect 0.656304 (fullspace_iso_3d_triangle/_test/triangle0_kernels_test.cc:24-40)
+  * closed-form crack solutions: Griffith crack (mean |t_n - 1| < 0.05,
+    elastic_square_matrix_generator_test.cc:134-182) and penny crack
+    w = 8 (1 - nu^2) / (pi E) sqrt(a^2 - r^2) -> t_n = 1;
+  * additivity of the influence integral: a rectangle equals the sum of its
+    two triangles (independent closed forms of 3DR0 and 3DT0);
+  * block counts of src/core/_test/hierarchical_representation_test.cc:19-93;
+  * the exact permutation of the reference clustering, replayed in Python on
+    top of the oracle's literal exchange sort, and - when the reference
+    harness oracle/_ref is built - against the compiled reference itself;
+  * ACA accuracy against the exact dense operator.
+"""
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle as O
+from bigwham_dev_amd import HostHmat
+from bigwham_dev_amd import _native as N
+from bigwham_dev_amd.meshes import penny_delaunay, rect_grid, segment_line
+
+ROOT = Path(__file__).resolve().parents[1]
+REF = ROOT / "oracle" / "_ref" / "libbwref.so"
+
+
+def test_2ds0_self_effect():
+    h = HostHmat.from_mesh("2DS0-H", np.array([[0.0, 0.0], [1.0, 0.0]]), np.array([[0, 1]]),
+                           [1.0, 0.3])
+    h.build_pattern(1, 2.0)
+    M = h.dense()
+    assert abs(M[0, 0] - 0.34979115367667662) < 1e-6
+    assert abs(M[1, 1] - 0.34979125861394933) < 1e-6
+    assert abs(M[0, 1]) < 1e-6 and abs(M[1, 0]) < 1e-6
+
+
+def _rect_1x2():
+    xyz = np.zeros((4, 3))
+    xyz[1, 0] = 1.0
+    xyz[2, 0] = 1.0
+    xyz[2, 1] = 2.0
+    xyz[3, 1] = 2.0
+    return xyz, np.array([[0, 1, 2, 3]])
+
+
+def test_3dr0_self_effect():
+    xyz, conn = _rect_1x2()
+    h = HostHmat.from_mesh("3DR0-H", xyz, conn, [1.0, 0.3])
+    t = h.influence(0, 0).T.ravel()  # column-major 3x3 as the reference's vector
+    assert abs(t[8] - 0.391078) < 1e-4
+    hm = HostHmat.from_mesh("3DR0-H-mode1", xyz, conn, [1.0, 0.3])
+    assert abs(hm.influence(0, 0)[0, 0] - 0.391078) < 1e-4
+
+
+def test_3dt0_self_effect():
+    xyz = np.array([[0.0, 0, 0], [1, 0, 0], [0, 1, 0]])
+    h = HostHmat.from_mesh("3DT0-H", xyz, np.array([[0, 1, 2]]), [1.0, 0.3])
+    assert abs(h.influence(0, 0)[0, 0] - 0.656304) < 1e-3
+    assert abs(h.influence(0, 0)[0, 0] - 0.656304) < 1e-6
+
+
+def _rotations(h, ne):
+    out = np.empty(ne * 9)
+    N.check_host(N.host().bwh_rotation_matrices(h._h, N.dptr(out)))
+    return out.reshape(ne, 3, 3)
+
+
+def test_rectangle_is_two_triangles():
+    a, b = 0.7, 0.4
+    A, B, Cc, D = [-a, -b, 0], [a, -b, 0], [a, b, 0], [-a, b, 0]
+    rng = np.random.default_rng(0)
+    for _ in range(6):
+        P = rng.uniform(-2, 2, 3)
+        P[2] = rng.uniform(0.05, 1.5) * rng.choice([-1, 1])
+        u = rng.normal(size=3)
+        u /= np.linalg.norm(u)
+        w = np.cross(u, rng.normal(size=3))
+        w /= np.linalg.norm(w)
+        tri = [P + 0.01 * u, P - 0.005 * u + 0.01 * w, P - 0.005 * u - 0.01 * w]
+        rect = [P + 0.01 * (-u - w), P + 0.01 * (u - w), P + 0.01 * (u + w), P + 0.01 * (-u + w)]
+        hr = HostHmat.from_mesh("3DR0-H", np.array([A, B, Cc, D, *rect]),
+                                np.array([[0, 1, 2, 3], [4, 5, 6, 7]]), [1.0, 0.25])
+        ht = HostHmat.from_mesh("3DT0-H", np.array([A, B, Cc, D, *tri]),
+                                np.array([[0, 1, 2], [0, 2, 3], [4, 5, 6]]), [1.0, 0.25])
+        RR, RT = _rotations(hr, 2), _rotations(ht, 3)
+        g_rect = RR[1].T @ hr.influence(0, 1) @ RR[0]
+        g_tri = (RT[2].T @ ht.influence(0, 2) @ RT[0] + RT[2].T @ ht.influence(1, 2) @ RT[1])
+        # the triangle kernel shifts the receiver by sqrt(eps) ~ 1.5e-8
+        assert np.linalg.norm(g_rect - g_tri) <= 1e-6 * np.linalg.norm(g_rect)
+
+
+def test_griffith_crack():
+    coor, conn = segment_line(1200)
+    h = HostHmat.from_mesh("2DS0-H", coor, conn, [1.0, 0.0])
+    h.build(32, 2.0, 1e-3)
+    xc = h.collocation_points()[:, 0]
+    x = np.zeros(h.size)
+    x[1::2] = 4.0 * np.sqrt(1.0 - xc ** 2)
+    y = O.matvec_original(h.flat(), x)
+    assert np.mean(np.abs(y[1::2] - 1.0)) < 0.05
+
+
+def test_penny_crack():
+    E, nu = 1.0, 0.25
+    coor, conn = penny_delaunay(1500, seed=3)
+    h = HostHmat.from_mesh("3DT0-H", coor, conn, [E, nu])
+    h.build(64, 3.0, 1e-6)
+    xc = h.collocation_points()
+    r = np.hypot(xc[:, 0], xc[:, 1])
+    x = np.zeros(h.size)
+    x[2::3] = 8 * (1 - nu ** 2) / (np.pi * E) * np.sqrt(np.maximum(1 - r ** 2, 0.0))
+    y = O.matvec_original(h.flat(), x)
+    inner = r < 0.8
+    assert np.mean(np.abs(y[2::3][inner] - 1.0)) < 0.06
+    assert np.max(np.abs(y[0::3])) < 1e-3 and np.max(np.abs(y[1::3])) < 1e-3
+
+
+def _line(n, h, offset_from=None, offset=0.0):
+    xs = np.arange(n + 1) * h
+    return np.stack([xs, np.zeros_like(xs)], 1), np.stack([np.arange(n), np.arange(1, n + 1)], 1)
+
+
+def test_block_counts():
+    coor, conn = _line(10, 0.123)
+    h = HostHmat.from_mesh("2DS0-H", coor, conn, [1.0, 0.25])
+    h.build_pattern(32, 1.0)
+    assert h.flat().n_fr == 1 and h.flat().n_lr == 0
+    coor, conn = _line(100, 0.123)
+    h = HostHmat.from_mesh("2DS0-H", coor, conn, [1.0, 0.25])
+    h.build_pattern(50, 0.0)
+    assert h.flat().n_fr == 4 and h.flat().n_lr == 0
+    n = 100
+    coor = np.zeros((n + 2, 2))
+    coor[: n // 2 + 1, 0] = np.arange(n // 2 + 1) * 0.1
+    coor[n // 2 + 1:, 0] = np.arange(n // 2 + 1, n + 2) * 0.1 + 60000.0
+    conn = np.array([[i, i + 1] for i in range(n // 2)] + [[i + 1, i + 2] for i in range(n // 2, n)])
+    h = HostHmat.from_mesh("2DS0-H", coor, conn, [1.0, 0.25])
+    h.build_pattern(n // 2, 4.0)
+    assert h.flat().n_fr == 2 and h.flat().n_lr == 2
+
+
+def _reference_cluster_py(pts, leaf, homogeneous):
+    """The reference's cluster_rec / cluster_rec_size_conservative
+    (cluster.cc:436-718) in Python, using the oracle's literal exchange sort."""
+    pts = np.ascontiguousarray(pts, dtype=np.float64).copy()
+    n, dim = pts.shape
+    perm = np.arange(n, dtype=np.int64)
+
+    def rec(b, e):
+        if e - b <= leaf:
+            return
+        wmax, dmax = 0.0, -1
+        for d in range(dim):
+            w = pts[b:e, d].max() - pts[b:e, d].min()
+            if w >= wmax:
+                wmax, dmax = w, d
+        O.exchange_sort(pts, perm, b, e, dmax)
+        if homogeneous:
+            k = 0
+            while leaf * 2.0 ** (k + 1) <= (e - b):
+                k += 1
+            card = (e - b) // 2 if (e - b) == leaf * 2.0 ** k else int(leaf * 2.0 ** k)
+        else:
+            card = (b + e) // 2 - b
+        rec(b, b + card)
+        rec(b + card, e)
+
+    rec(0, n)
+    return perm
+
+
+@pytest.mark.parametrize("homogeneous", [False, True])
+@pytest.mark.parametrize("mesh", ["line", "grid", "ties"])
+def test_cluster_permutation_matches_reference_algorithm(mesh, homogeneous):
+    if mesh == "line":
+        coor, conn = segment_line(700)
+        kernel, leaf = "2DS0-H", 32
+    elif mesh == "grid":
+        coor, conn = rect_grid(30, 27)
+        kernel, leaf = "3DR0-H", 64
+    else:
+        # unstructured triangles on a coarse lattice: many exact coordinate ties
+        rng = np.random.default_rng(2)
+        coor, conn = penny_delaunay(900, seed=1)
+        coor = np.round(coor * 8) / 8
+        a, b, c = coor[conn[:, 0]], coor[conn[:, 1]], coor[conn[:, 2]]
+        keep = np.abs(np.cross(b - a, c - a)[:, 2]) > 1e-9
+        conn = conn[keep]
+        kernel, leaf = "3DT0-H", 64
+        del rng
+    h = HostHmat.from_mesh(kernel, coor, conn, [1.0, 0.25], homogeneous_size_pattern=homogeneous)
+    h.build_pattern(leaf, 3.0)
+    perm_ref = _reference_cluster_py(h.collocation_points(), leaf, homogeneous)
+    assert np.array_equal(h.permutation(), perm_ref)
+
+
+@pytest.mark.skipif(not REF.exists(), reason="reference harness oracle/_ref not built")
+@pytest.mark.parametrize("homogeneous", [0, 1])
+def test_pattern_matches_compiled_reference(homogeneous):
+    ref = C.CDLL(str(REF))
+    D, I = C.POINTER(C.c_double), C.POINTER(C.c_int64)
+    ref.ref_pattern.argtypes = [D, C.c_int64, C.c_int, C.c_int64, C.c_double, C.c_int, C.c_int,
+                                C.c_int64, I, I, I, I, I]
+    for kernel, (coor, conn), leaf in [("2DS0-H", segment_line(2000), 32),
+                                       ("3DR0-H", rect_grid(60), 64),
+                                       ("3DT0-H", penny_delaunay(3000, seed=5), 64)]:
+        h = HostHmat.from_mesh(kernel, coor, conn, [1.0, 0.25],
+                               homogeneous_size_pattern=bool(homogeneous))
+        h.build_pattern(leaf, 3.0)
+        pts = np.ascontiguousarray(h.collocation_points())
+        n, dim = pts.shape
+        cap = 200000
+        fr = np.zeros(6 * cap, np.int64)
+        lr = np.zeros(6 * cap, np.int64)
+        nfr, nlr = C.c_int64(), C.c_int64()
+        perm = np.zeros(n, np.int64)
+        ref.ref_pattern(pts.ctypes.data_as(D), n, dim, leaf, 3.0, homogeneous, -1, cap,
+                        fr.ctypes.data_as(I), C.byref(nfr), lr.ctypes.data_as(I), C.byref(nlr),
+                        perm.ctypes.data_as(I))
+        assert np.array_equal(h.permutation(), perm)
+        assert np.array_equal(h.fr_pattern()[:5], fr[: 6 * nfr.value].reshape(-1, 6).T[:5])
+        assert np.array_equal(h.lr_pattern()[:5], lr[: 6 * nlr.value].reshape(-1, 6).T[:5])
+
+
+@pytest.mark.skipif(not REF.exists(), reason="reference harness oracle/_ref not built")
+def test_segment_kernel_matches_compiled_reference():
+    ref = C.CDLL(str(REF))
+    ref.ref_we_segment_0.argtypes = [C.c_double] * 5 + [C.POINTER(C.c_double)]
+    ref.ref_we_segment_0.restype = None
+    # compare through a two-element mesh: influence(src, rec) uses We_segment_0
+    rng = np.random.default_rng(4)
+    for _ in range(20):
+        p = rng.uniform(-1, 1, (4, 2))
+        h = HostHmat.from_mesh("2DS0-H", p, np.array([[0, 1], [2, 3]]), [1.3, 0.21])
+        R = _rotations2d(h)
+        src_c = p[:2].mean(0)
+        size = np.linalg.norm(p[1] - p[0])
+        xl = R[0] @ (p[2:].mean(0) - src_c)
+        W = np.zeros(6)
+        G = 1.3 / (2 * 1.21)
+        ref.ref_we_segment_0(size, G, 0.21, xl[0], xl[1], W.ctypes.data_as(C.POINTER(C.c_double)))
+        W = W.reshape(2, 3)
+        n = R[0] @ (R[1][1])
+        s = R[0] @ (R[1][0])
+        exp = np.array([[n[0] * s[0] * W[0, 0] + (n[0] * s[1] + n[1] * s[0]) * W[0, 1] + n[1] * s[1] * W[0, 2],
+                         n[0] * s[0] * W[1, 0] + (n[0] * s[1] + n[1] * s[0]) * W[1, 1] + n[1] * s[1] * W[1, 2]],
+                        [n[0] ** 2 * W[0, 0] + 2 * n[0] * n[1] * W[0, 1] + n[1] ** 2 * W[0, 2],
+                         n[0] ** 2 * W[1, 0] + 2 * n[0] * n[1] * W[1, 1] + n[1] ** 2 * W[1, 2]]])
+        got = h.influence(0, 1)
+        assert np.allclose(got, exp, rtol=1e-12, atol=1e-12 * np.abs(exp).max())
+
+
+def _rotations2d(h):
+    out = np.empty(2 * 4)
+    N.check_host(N.host().bwh_rotation_matrices(h._h, N.dptr(out)))
+    return out.reshape(2, 2, 2)
+
+
+@pytest.mark.parametrize("kernel", ["2DS0-H", "3DR0-H", "3DR0-H-mode1", "3DT0-H"])
+def test_aca_accuracy_against_dense(kernel):
+    if kernel == "2DS0-H":
+        coor, conn = segment_line(600)
+        leaf = 32
+    elif kernel == "3DT0-H":
+        coor, conn = penny_delaunay(500, seed=2)
+        leaf = 64
+    else:
+        coor, conn = rect_grid(20)
+        leaf = 64
+    h = HostHmat.from_mesh(kernel, coor, conn, [1.0, 0.25])
+    h.build(leaf, 3.0, 1e-6)
+    f = h.flat()
+    assert f.n_lr > 0
+    Hd = O.dense(f)
+    M = h.dense()
+    err = np.linalg.norm(Hd - M) / np.linalg.norm(M)
+    assert err < 1e-5
+    assert 0.0 <= h.max_aca_error() < 1e-5 or h.max_aca_error() == -1.0
+
+
+def test_gaussian_operator():
+    """Hmat_test_gaussian_mat.cpp: builds, and sum of the diagonal == n exactly."""
+    for n, alpha in [(100, 1.0), (150, 10.0)]:
+        h = HostHmat.gaussian(n, alpha)
+        h.build(32, 3.0, 1e-3)
+        assert O.diagonal(h.flat()).sum() == 1.0 * n

@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -28,6 +29,14 @@
 namespace {
 
 thread_local std::string g_err;
+
+bool verbose() {
+  static const bool v = [] {
+    const char* e = std::getenv("BIGWHAM_VERBOSE");
+    return e && e[0] && e[0] != '0';
+  }();
+  return v;
+}
 
 int fail(int code, const std::string& m) {
   g_err = m;
@@ -318,6 +327,8 @@ int upload_pass(const bwl::Layout& L, const bwl::Program& P, DevPass* dp, hipStr
     hipEventRecord(done[buf], st);
     pending[buf] = true;
     buf ^= 1;
+    if (verbose() && nslab >= 10 && (e * 10 / nslab) != (s * 10 / nslab))
+      std::fprintf(stderr, "[bwhip] upload %d / %d slabs\n", e, nslab);
     s = e;
   }
   hipStreamSynchronize(st);

@@ -263,6 +263,11 @@ int bwh_build_pattern(bwh_hmat* h, int max_leaf_size, double eta) {
   h->pattern_built = true;
   h->built = false;
   h->pattern_time = now_s() - t0;
+  if (verbose())
+    std::fprintf(stderr, "[bwh] pattern: %zu full-rank, %zu low-rank blocks, %.2f s "
+                 "(cluster sorts with tie order not replayed: %lld)\n",
+                 fr.size(), lr.size(), h->pattern_time,
+                 static_cast<long long>(bwh::inexact_sorts().load()));
   return BW_OK;
 }
 
@@ -302,9 +307,24 @@ int bwh_build_hmat(bwh_hmat* h, int max_leaf_size, double eta, double eps_aca) {
   // LR: ACA per block
   h->lr_blocks.assign(nlr, bwh::LowRankBlock{});
   const int fixed = h->fixed_rank;
-  parallel_for(h->nthreads, nlr, 1, [&](int64_t k) {
+  // largest blocks first, so the tail of the parallel loop holds small ones
+  std::vector<int64_t> order(nlr);
+  for (int64_t k = 0; k < nlr; ++k) order[k] = k;
+  std::sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
+    const int64_t* ca = &h->lr_pattern[6 * a];
+    const int64_t* cb = &h->lr_pattern[6 * b];
+    return (ca[3] - ca[1]) + (ca[4] - ca[2]) > (cb[3] - cb[1]) + (cb[4] - cb[2]);
+  });
+  std::atomic<int64_t> done{0};
+  const double t_lr0 = now_s();
+  parallel_for(h->nthreads, nlr, 1, [&](int64_t i) {
+    const int64_t k = order[i];
     const int64_t* c = &h->lr_pattern[6 * k];
     h->lr_blocks[k] = bwh::aca(*gen, c[1], c[3], c[2], c[4], eps_aca, fixed);
+    const int64_t d = done.fetch_add(1) + 1;
+    if (verbose() && nlr >= 10 && d % (nlr / 10) == 0)
+      std::fprintf(stderr, "[bwh] ACA %lld / %lld blocks, %.1f s\n", static_cast<long long>(d),
+                   static_cast<long long>(nlr), now_s() - t_lr0);
   });
   h->lr_A.resize(nlr);
   h->lr_B.resize(nlr);

@@ -19,6 +19,7 @@
 // pattern_rec (src/hmat/hmatrix/h_pattern.cc:87-142) with the tree spot index
 // equal to the il::Tree append order (il/il/Tree.h AddChild).
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <limits>
 #include <thread>
@@ -93,14 +94,23 @@ void exchange_sort_rows(std::vector<double>& pts, int dim,
   }
   // Ties: replay the exchange sort pass by pass.  Pass i swaps position i
   // with each later position holding a value strictly below the value
-  // currently at i; a min segment tree finds those positions.
+  // currently at i; a min segment tree finds those positions, which is cheap
+  // on structured (grid) coordinates.  On scattered coordinates the number of
+  // swaps grows like n^2/4: past a swap budget the replay continues with the
+  // literal double loop for clusters up to kExactLimit points, and above that
+  // finishes the remaining passes with a stable sort of the current
+  // arrangement, which is exact for every distinct key and keeps exactly tied
+  // points in their current relative order.
   MinTree tree(key);
   auto swap_rows = [&](int64_t i, int64_t k) {
     for (int q = 0; q < dim; ++q) std::swap(pts[(b + i) * dim + q], pts[(b + k) * dim + q]);
     std::swap(perm[b + i], perm[b + k]);
     std::swap(key[i], key[k]);
   };
-  for (int64_t i = 0; i < n; ++i) {
+  const int64_t budget = 32 * n * (1 + static_cast<int64_t>(std::log2(static_cast<double>(n))));
+  int64_t swaps = 0;
+  int64_t i = 0;
+  for (; i < n && swaps <= budget; ++i) {
     int64_t from = i + 1;
     while (true) {
       const int64_t q = tree.first_less(from, key[i]);
@@ -108,9 +118,34 @@ void exchange_sort_rows(std::vector<double>& pts, int dim,
       swap_rows(i, q);
       tree.update(q, key[q]);
       from = q + 1;
+      ++swaps;
     }
     tree.update(i, key[i]);
   }
+  if (i >= n) return;
+  if (n <= kExactLimit) {
+    for (; i < n; ++i)
+      for (int64_t k = i + 1; k < n; ++k)
+        if (key[i] > key[k]) swap_rows(i, k);
+    return;
+  }
+  inexact_sorts().fetch_add(1);
+  std::vector<int64_t> idx(n - i);
+  for (int64_t j = i; j < n; ++j) idx[j - i] = j;
+  std::stable_sort(idx.begin(), idx.end(), [&](int64_t a, int64_t c) { return key[a] < key[c]; });
+  std::vector<double> rows(static_cast<size_t>(n - i) * dim);
+  std::vector<int64_t> pm(n - i);
+  for (int64_t j = 0; j < n - i; ++j) {
+    for (int q = 0; q < dim; ++q) rows[j * dim + q] = pts[(b + idx[j]) * dim + q];
+    pm[j] = perm[b + idx[j]];
+  }
+  std::copy(rows.begin(), rows.end(), pts.begin() + (b + i) * dim);
+  std::copy(pm.begin(), pm.end(), perm.begin() + b + i);
+}
+
+std::atomic<int64_t>& inexact_sorts() {
+  static std::atomic<int64_t> count{0};
+  return count;
 }
 
 namespace {

@@ -7,6 +7,7 @@
 // the centroid for piece-wise constant elements.
 #pragma once
 
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <memory>
@@ -94,9 +95,14 @@ struct ClusterTree {
 ClusterTree build_cluster_tree(int64_t leaf_size, std::vector<double>& pts,
                                int dim, bool homogeneous_size);
 
-// Stable reference order of the exchange sort on [b, e) by coordinate d.
+// The reference's exchange sort on rows [b, e) by coordinate d.  Exact for
+// distinct keys and for ties in clusters of up to kExactLimit points; larger
+// clusters with exactly tied keys keep the ties in a deterministic order that
+// may differ from the reference's (counted by inexact_sorts()).
+constexpr int64_t kExactLimit = 60000;
 void exchange_sort_rows(std::vector<double>& pts, int dim,
                         std::vector<int64_t>& perm, int64_t b, int64_t e, int d);
+std::atomic<int64_t>& inexact_sorts();
 
 struct BlockEntry {
   int64_t spot, i0, j0, i1, j1;

@@ -94,7 +94,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--config", default=os.environ.get("BW_BENCH_CONFIG", "c2"))
+    ap.add_argument("--config", default=os.environ.get("BW_BENCH_CONFIG", "c3"))
     ap.add_argument("--gmres", action="store_true", help="also time GMRES to 1e-8")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--wi-bytes", type=int, default=0)

@@ -107,7 +107,9 @@ __global__ __launch_bounds__(256) void k_slab(const bwl::WorkItem* __restrict__ 
                                               const T* __restrict__ data,
                                               const T* __restrict__ xin,
                                               OutT* __restrict__ out, T* __restrict__ part,
-                                              const int64_t* __restrict__ perm, int dof) {
+                                              const int64_t* __restrict__ perm, int dof,
+                                              const bwl::Reduce* __restrict__ reds,
+                                              int* __restrict__ cnt) {
   typedef typename Vec<T>::type VT;
   constexpr int V = Vec<T>::V;
   const int lane = threadIdx.x & 63;
@@ -190,14 +192,32 @@ __global__ __launch_bounds__(256) void k_slab(const bwl::WorkItem* __restrict__ 
     const T o = __shfl(acc, lane + s * R, 64);
     if (split == 0) acc += o;
   }
-  if (lane < R) {
-    if (wi.part_off >= 0) {
-      part[wi.part_off + lane] = acc;
-    } else {
+  if (wi.part_off < 0) {
+    if (lane < R)
       out[out_index<OutT>(static_cast<int64_t>(wi.out_row) + lane, perm, dof)] =
           static_cast<OutT>(acc);
-    }
+    return;
   }
+  if (lane < R) part[wi.part_off + lane] = acc;
+  if (wi.red < 0) return;  // summed by the separate reduce pass
+  // Fused reduction: the last work item of the slab to finish adds the
+  // partials of all its work items, in work-item order.
+  __threadfence();
+  int ticket = 0;
+  if (lane == 0) ticket = atomicAdd(&cnt[wi.red], 1);
+  ticket = __shfl(ticket, 0, 64);
+  const bwl::Reduce rd = reds[wi.red];
+  if (ticket != rd.nparts - 1) return;
+  __threadfence();
+  if (lane < R) {
+    T s = 0;
+    const T* pp = part + rd.part_off + lane;
+    for (int p = 0; p < rd.nparts; ++p)
+      s += __hip_atomic_load(pp + p * R, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    out[out_index<OutT>(static_cast<int64_t>(rd.out_row) + lane, perm, dof)] =
+        static_cast<OutT>(s);
+  }
+  if (lane == 0) cnt[wi.red] = 0;  // ready for the next launch
 }
 
 template <typename T, typename OutT>
@@ -219,10 +239,12 @@ __global__ __launch_bounds__(64) void k_reduce(const bwl::Reduce* __restrict__ r
 struct DevPass {
   bwl::WorkItem* wis = nullptr;
   int32_t* gidx = nullptr;
-  bwl::Reduce* reds = nullptr;
+  bwl::Reduce* reds = nullptr;      // fused reductions
+  bwl::Reduce* big_reds = nullptr;  // separate reduce pass
+  int* cnt = nullptr;               // per fused reduction, zero between launches
   void* data = nullptr;
   void* part = nullptr;
-  int nwi = 0, nred = 0;
+  int nwi = 0, nred = 0, nbig = 0;
   int64_t data_bytes = 0;
 };
 
@@ -246,6 +268,7 @@ int upload_pass(const bwl::Layout& L, const bwl::Program& P, DevPass* dp, hipStr
                 std::vector<void*>* allocs) {
   dp->nwi = static_cast<int>(P.wis.size());
   dp->nred = static_cast<int>(P.reds.size());
+  dp->nbig = static_cast<int>(P.big_reds.size());
   auto dalloc = [&](void** p, size_t bytes) -> int {
     if (bytes == 0) bytes = 16;
     hipError_t e = hipMalloc(p, bytes);
@@ -259,6 +282,11 @@ int upload_pass(const bwl::Layout& L, const bwl::Program& P, DevPass* dp, hipStr
   if ((rc = dalloc(reinterpret_cast<void**>(&dp->gidx), P.gidx.size() * sizeof(int32_t)))) return rc;
   if ((rc = dalloc(reinterpret_cast<void**>(&dp->reds), P.reds.size() * sizeof(bwl::Reduce))))
     return rc;
+  if ((rc = dalloc(reinterpret_cast<void**>(&dp->big_reds),
+                   P.big_reds.size() * sizeof(bwl::Reduce))))
+    return rc;
+  if ((rc = dalloc(reinterpret_cast<void**>(&dp->cnt), P.reds.size() * sizeof(int)))) return rc;
+  HIPCHK(hipMemset(dp->cnt, 0, std::max<size_t>(P.reds.size(), 1) * sizeof(int)));
   if ((rc = dalloc(&dp->part, static_cast<size_t>(P.part_elems) * sizeof(T)))) return rc;
   dp->data_bytes = P.data_elems * static_cast<int64_t>(sizeof(T));
   if ((rc = dalloc(&dp->data, static_cast<size_t>(dp->data_bytes)))) return rc;
@@ -268,6 +296,9 @@ int upload_pass(const bwl::Layout& L, const bwl::Program& P, DevPass* dp, hipStr
     HIPCHK(hipMemcpy(dp->gidx, P.gidx.data(), P.gidx.size() * sizeof(int32_t), hipMemcpyHostToDevice));
   if (!P.reds.empty())
     HIPCHK(hipMemcpy(dp->reds, P.reds.data(), P.reds.size() * sizeof(bwl::Reduce), hipMemcpyHostToDevice));
+  if (!P.big_reds.empty())
+    HIPCHK(hipMemcpy(dp->big_reds, P.big_reds.data(), P.big_reds.size() * sizeof(bwl::Reduce),
+                     hipMemcpyHostToDevice));
   // data: fill slabs into pinned staging chunks and stream them up
   const int32_t nslab = static_cast<int32_t>(P.slabs.size());
   if (nslab == 0) return BW_OK;
@@ -441,23 +472,24 @@ int launch_matvec(bw_hip_handle* hd, DevOp& op, const double* xd, double* yd, in
     const int blocks = (op.p1.nwi + 3) / 4;
     hipLaunchKernelGGL((k_slab<T, T, 8, 1>), dim3(blocks), dim3(256), 0, st, op.p1.wis, op.p1.nwi,
                        op.p1.gidx, static_cast<const T*>(op.p1.data), xt, xt,
-                       static_cast<T*>(op.p1.part), nullptr, dof);
+                       static_cast<T*>(op.p1.part), nullptr, dof, op.p1.reds, op.p1.cnt);
   }
   if (hd->timing) hipEventRecord(hd->ev[2], st);
-  if (op.p1.nred > 0)
-    hipLaunchKernelGGL((k_reduce<T, T>), dim3(op.p1.nred), dim3(64), 0, st, op.p1.reds,
-                       op.p1.nred, static_cast<const T*>(op.p1.part), xt, nullptr, dof);
+  if (op.p1.nbig > 0)
+    hipLaunchKernelGGL((k_reduce<T, T>), dim3(op.p1.nbig), dim3(64), 0, st, op.p1.big_reds,
+                       op.p1.nbig, static_cast<const T*>(op.p1.part), xt, nullptr, dof);
   if (hd->timing) hipEventRecord(hd->ev[3], st);
   if (op.p2.nwi > 0) {
     const int blocks = (op.p2.nwi + 3) / 4;
     hipLaunchKernelGGL((k_slab<T, double, 8, 2>), dim3(blocks), dim3(256), 0, st, op.p2.wis,
                        op.p2.nwi, op.p2.gidx, static_cast<const T*>(op.p2.data), xt, yd,
-                       static_cast<T*>(op.p2.part), orig ? op.perm_out : nullptr, dof);
+                       static_cast<T*>(op.p2.part), orig ? op.perm_out : nullptr, dof,
+                       op.p2.reds, op.p2.cnt);
   }
   if (hd->timing) hipEventRecord(hd->ev[4], st);
-  if (op.p2.nred > 0)
-    hipLaunchKernelGGL((k_reduce<T, double>), dim3(op.p2.nred), dim3(64), 0, st, op.p2.reds,
-                       op.p2.nred, static_cast<const T*>(op.p2.part), yd,
+  if (op.p2.nbig > 0)
+    hipLaunchKernelGGL((k_reduce<T, double>), dim3(op.p2.nbig), dim3(64), 0, st, op.p2.big_reds,
+                       op.p2.nbig, static_cast<const T*>(op.p2.part), yd,
                        orig ? op.perm_out : nullptr, dof);
   if (hd->timing) hipEventRecord(hd->ev[5], st);
   HIPCHK(hipGetLastError());

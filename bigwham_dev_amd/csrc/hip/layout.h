@@ -48,16 +48,21 @@ struct WorkItem {
   int32_t rows;      // R
   int32_t out_row;   // output row of the WI's row 0 (direct write)
   int32_t part_off;  // -1: direct write, else offset into the partial buffer
-  int32_t pad;
+  int32_t red;       // reduce entry of a split slab whose last WI sums the
+                     // partials (fused), or -1
 };
 
-// Reduce descriptor for a slab split over several WIs (16 bytes).
+// Reduce descriptor for a slab split over several WIs (16 bytes).  Slabs of
+// at most kFuseMaxParts WIs are summed by their last WI to finish (counted
+// with a device-side atomic); larger ones by a separate reduce pass.
 struct Reduce {
   int32_t out_row;
   int32_t rows;
   int32_t part_off;
   int32_t nparts;
 };
+
+constexpr int kFuseMaxParts = 48;
 
 struct Piece {
   int64_t block;   // index into the view's FR or LR arrays
@@ -86,7 +91,8 @@ struct SlabDesc {
 struct Program {
   std::vector<WorkItem> wis;
   std::vector<int32_t> gidx;
-  std::vector<Reduce> reds;
+  std::vector<Reduce> reds;       // fused reductions (indexed by WorkItem::red)
+  std::vector<Reduce> big_reds;   // reductions left to the separate pass
   std::vector<SlabDesc> slabs;
   std::vector<RunDesc> runs;
   std::vector<Piece> pieces;

@@ -9,11 +9,12 @@ torch FP64 vectors:
   gmres()        restarted right-preconditioned GMRES, FP64 Arnoldi with
                  classical Gram-Schmidt and one re-orthogonalisation pass,
                  Givens rotations on the host.
-  gmres_ir()     mixed precision: iterative refinement whose residual
+  solve_mixed()  mixed precision: iterative refinement whose residual
                  r = b - H x is evaluated with the FP64 matvec and whose
-                 correction solves H d = r by gmres() driven by the FP32-factor
-                 matvec (FP64 Arnoldi and vectors).  Stops on the FP64
-                 residual.
+                 correction solves H d = r by one restart cycle of gmres()
+                 driven by the FP32-factor matvec (FP64 Arnoldi and vectors),
+                 so every restart of the Krylov cycle starts from a true FP64
+                 residual.  Stops on the FP64 residual.
 """
 from __future__ import annotations
 
@@ -143,10 +144,16 @@ def solve_fp64(hmat, b: torch.Tensor, tol: float = 1e-8, restart: int = 30,
     return x, SolveInfo(rel <= tol, its, n64[0] + 1, 0, rel, dt)
 
 
-def solve_mixed(hmat, b: torch.Tensor, tol: float = 1e-8, inner_tol: float = 1e-4,
-                restart: int = 30, maxiter: int = 2000, max_refine: int = 20,
+def solve_mixed(hmat, b: torch.Tensor, tol: float = 1e-8, inner_tol: float = 1e-5,
+                restart: int = 30, maxiter: int = 2000, max_refine: int = 500,
                 precondition: bool = True) -> tuple[torch.Tensor, SolveInfo]:
-    """Iterative refinement: FP64 residuals, FP32-factor inner GMRES."""
+    """Iterative refinement: FP64 residuals, FP32-factor inner GMRES.
+
+    Each refinement step runs one restart cycle (at most `restart` FP32
+    matvecs) on H d = r, stopping early once the inner residual drops below
+    max(inner_tol, 0.5 tol / ||r||/||b||); the next step starts from the
+    exact FP64 residual.  The iteration count then follows the FP64 restarted
+    GMRES, with one extra FP64 matvec per cycle."""
     d = torch.from_numpy(hmat.diagonal(original=True)).to(b.device)
     M = jacobi(d) if precondition else None
     counts = {"fp64": 0, "fp32": 0}
@@ -164,7 +171,9 @@ def solve_mixed(hmat, b: torch.Tensor, tol: float = 1e-8, inner_tol: float = 1e-
     hist = []
     its = 0
     for _ in range(max_refine):
-        dx, k, _ = gmres(mv32, r, M, tol=inner_tol, restart=restart, maxiter=maxiter - its)
+        target = max(inner_tol, 0.5 * tol / rel)
+        dx, k, _ = gmres(mv32, r, M, tol=target, restart=restart,
+                         maxiter=min(restart, maxiter - its))
         its += k
         x = x + dx
         r = b - hmat.matvec(x, original=True, precision="fp64")

@@ -107,9 +107,7 @@ __global__ __launch_bounds__(256) void k_slab(const bwl::WorkItem* __restrict__ 
                                               const T* __restrict__ data,
                                               const T* __restrict__ xin,
                                               OutT* __restrict__ out, T* __restrict__ part,
-                                              const int64_t* __restrict__ perm, int dof,
-                                              const bwl::Reduce* __restrict__ reds,
-                                              int* __restrict__ cnt) {
+                                              const int64_t* __restrict__ perm, int dof) {
   typedef typename Vec<T>::type VT;
   constexpr int V = Vec<T>::V;
   const int lane = threadIdx.x & 63;
@@ -198,26 +196,11 @@ __global__ __launch_bounds__(256) void k_slab(const bwl::WorkItem* __restrict__ 
           static_cast<OutT>(acc);
     return;
   }
+  // Split slab: the partial sums are added in work-item order by k_reduce.
+  // (A last-work-item-reduces variant needs agent-scope release/acquire
+  // fences, which write back and invalidate the per-XCD L2 and cost more than
+  // the separate pass saves.)
   if (lane < R) part[wi.part_off + lane] = acc;
-  if (wi.red < 0) return;  // summed by the separate reduce pass
-  // Fused reduction: the last work item of the slab to finish adds the
-  // partials of all its work items, in work-item order.
-  __threadfence();
-  int ticket = 0;
-  if (lane == 0) ticket = atomicAdd(&cnt[wi.red], 1);
-  ticket = __shfl(ticket, 0, 64);
-  const bwl::Reduce rd = reds[wi.red];
-  if (ticket != rd.nparts - 1) return;
-  __threadfence();
-  if (lane < R) {
-    T s = 0;
-    const T* pp = part + rd.part_off + lane;
-    for (int p = 0; p < rd.nparts; ++p)
-      s += __hip_atomic_load(pp + p * R, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    out[out_index<OutT>(static_cast<int64_t>(rd.out_row) + lane, perm, dof)] =
-        static_cast<OutT>(s);
-  }
-  if (lane == 0) cnt[wi.red] = 0;  // ready for the next launch
 }
 
 template <typename T, typename OutT>
@@ -229,9 +212,141 @@ __global__ __launch_bounds__(64) void k_reduce(const bwl::Reduce* __restrict__ r
   const bwl::Reduce rd = reds[r];
   const int t = threadIdx.x;
   if (t >= rd.rows) return;
-  T s = 0;
-  for (int p = 0; p < rd.nparts; ++p) s += part[rd.part_off + p * rd.rows + t];
+  // four interleaved partial sums keep four loads in flight; the order of
+  // the additions is fixed, so the result is deterministic
+  const T* p = part + rd.part_off + t;
+  const int R = rd.rows;
+  T s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+  int q = 0;
+  for (; q + 4 <= rd.nparts; q += 4) {
+    s0 += p[(q + 0) * R];
+    s1 += p[(q + 1) * R];
+    s2 += p[(q + 2) * R];
+    s3 += p[(q + 3) * R];
+  }
+  for (; q < rd.nparts; ++q) s0 += p[q * R];
+  const T s = (s0 + s1) + (s2 + s3);
   out[out_index<OutT>(static_cast<int64_t>(rd.out_row) + t, perm, dof)] = static_cast<OutT>(s);
+}
+
+// ---------------------------------------------------------------------------
+// Many right-hand sides on FP32 MFMA.
+//
+// The FP32 slab layout serves as the A operand of v_mfma_f32_32x32x2f32
+// without change: lanes 0-31 read rows 0-31 of column group g, lanes 32-63
+// rows 0-31 of group g+1, 16 bytes (4 columns) each; element s of a lane's
+// vector is A[row][k] of MFMA step s, with k = lane / 32 selecting the group.
+// The B operand is the gathered input XB[z][n] (row-major, kRhs columns):
+// lane l reads XB[z(g + l/32) + s][l % 32], 128 contiguous bytes per
+// half-wavefront.  The 32 x 32 accumulator holds rows x right-hand sides;
+// element j of lane l is row 8*(j/4) + 4*(l/32) + j%4, column l % 32.
+constexpr int kRhs = 32;
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+__global__ __launch_bounds__(256) void k_gather_b(const double* __restrict__ X, int64_t ldx,
+                                                  int nrhs, float* __restrict__ XB, int64_t n,
+                                                  const int64_t* __restrict__ perm, int dof) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * (blockDim.x / kRhs) + threadIdx.x / kRhs;
+  const int c = threadIdx.x % kRhs;
+  if (i >= n) return;
+  int64_t src = i;
+  if (perm) {
+    const int64_t p = i / dof;
+    src = dof * perm[p] + (i - p * dof);
+  }
+  XB[i * kRhs + c] = c < nrhs ? static_cast<float>(X[src + c * ldx]) : 0.0f;
+}
+
+template <int PASS>
+__global__ __launch_bounds__(256) void k_slab_mfma(const bwl::WorkItem* __restrict__ wis, int nwi,
+                                                   const int32_t* __restrict__ gidx,
+                                                   const float* __restrict__ data,
+                                                   const float* __restrict__ XB,
+                                                   float* __restrict__ tout,      // pass 1
+                                                   double* __restrict__ Y, int64_t ldy,  // pass 2
+                                                   int nrhs, float* __restrict__ part,
+                                                   const int64_t* __restrict__ perm, int dof,
+                                                   const bwl::Reduce* __restrict__ reds,
+                                                   int* __restrict__ cnt) {
+  const int lane = threadIdx.x & 63;
+  const int w = xcd_remap(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
+  if (w >= nwi) return;
+  const bwl::WorkItem wi = wis[w];
+  const int R = wi.rows;
+  const int ng = wi.ngroups;
+  const int half = lane >> 5;  // which group of the pair
+  const int r0 = lane & 31;
+  const flt4* __restrict__ base = reinterpret_cast<const flt4*>(data + wi.data_off);
+  const int32_t* __restrict__ gi = gidx + wi.gbeg;
+  floatx16 acc0 = {0}, acc1 = {0};
+  const bool two = R > 32;
+  for (int g = 0; g < ng; g += 2) {
+    const int gg = g + half;
+    const bool has = gg < ng;
+    flt4 a0 = {0.f, 0.f, 0.f, 0.f}, a1 = {0.f, 0.f, 0.f, 0.f};
+    int z = 0;
+    if (has) {
+      z = gi[gg];
+      if (r0 < R) a0 = __builtin_nontemporal_load(base + gg * R + r0);
+      if (two && r0 + 32 < R) a1 = __builtin_nontemporal_load(base + gg * R + 32 + r0);
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const float b = has ? XB[static_cast<int64_t>(z + s) * kRhs + r0] : 0.0f;
+      acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0[s], b, acc0, 0, 0, 0);
+      if (two) acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1[s], b, acc1, 0, 0, 0);
+    }
+  }
+  const int n = lane & 31;
+  auto row_of = [&](int j) { return 8 * (j >> 2) + 4 * half + (j & 3); };
+  auto store = [&](int row, float v, int64_t out_row) {
+    if (row >= R || n >= nrhs) return;
+    if (PASS == 1) {
+      tout[(out_row + row) * kRhs + n] = v;
+    } else {
+      Y[out_index<double>(out_row + row, perm, dof) + n * ldy] = static_cast<double>(v);
+    }
+  };
+  if (wi.part_off < 0) {
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      store(row_of(j), acc0[j], wi.out_row);
+      if (two) store(32 + row_of(j), acc1[j], wi.out_row);
+    }
+    return;
+  }
+  float* pp = part + static_cast<int64_t>(wi.part_off) * kRhs;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    const int row = row_of(j);
+    if (row < R) pp[row * kRhs + n] = acc0[j];
+    if (two && 32 + row < R) pp[(32 + row) * kRhs + n] = acc1[j];
+  }
+  // partial sums of split slabs are added by k_reduce_b
+  (void)reds;
+  (void)cnt;
+}
+
+template <int PASS>
+__global__ __launch_bounds__(64) void k_reduce_b(const bwl::Reduce* __restrict__ reds, int nred,
+                                                 const float* __restrict__ part, float* tout,
+                                                 double* Y, int64_t ldy, int nrhs,
+                                                 const int64_t* __restrict__ perm, int dof) {
+  const int r = blockIdx.x;
+  if (r >= nred) return;
+  const bwl::Reduce rd = reds[r];
+  const float* base_p = part + static_cast<int64_t>(rd.part_off) * kRhs;
+  for (int e = threadIdx.x; e < rd.rows * kRhs; e += 64) {
+    float s = 0.0f;
+    for (int q = 0; q < rd.nparts; ++q) s += base_p[static_cast<int64_t>(q) * rd.rows * kRhs + e];
+    const int row = e / kRhs, nn = e % kRhs;
+    if (nn >= nrhs) continue;
+    if (PASS == 1)
+      tout[(static_cast<int64_t>(rd.out_row) + row) * kRhs + nn] = s;
+    else
+      Y[out_index<double>(static_cast<int64_t>(rd.out_row) + row, perm, dof) + nn * ldy] =
+          static_cast<double>(s);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -258,6 +373,10 @@ struct DevOp {
   const int64_t* perm_in = nullptr;   // device, n_in / dof
   const int64_t* perm_out = nullptr;  // device, n_out / dof
   int64_t stream_bytes = 0;
+  int64_t part_elems[2] = {0, 0};
+  // many right-hand sides (FP32 MFMA path), allocated on first use
+  float* xb = nullptr;        // xt_size x kRhs
+  float* partb[2] = {nullptr, nullptr};
 };
 
 std::atomic<int64_t> g_wi_bytes64{96 * 1024};
@@ -424,6 +543,8 @@ int build_op(bw_hip_handle* hd, const bw_flat_hmat* h, bool adjoint, hipStream_t
   op.perm_in = adjoint ? hd->perm0 : hd->perm1;
   op.perm_out = adjoint ? hd->perm1 : hd->perm0;
   op.stream_bytes = (L.pass1.data_elems + L.pass2.data_elems) * static_cast<int64_t>(sizeof(T));
+  op.part_elems[0] = L.pass1.part_elems;
+  op.part_elems[1] = L.pass2.part_elems;
   hd->device_bytes += op.p1.data_bytes + op.p2.data_bytes + L.xt_size * sizeof(T) +
                       (L.pass1.part_elems + L.pass2.part_elems) * sizeof(T) +
                       (L.pass1.wis.size() + L.pass2.wis.size()) * sizeof(bwl::WorkItem) +
@@ -472,7 +593,7 @@ int launch_matvec(bw_hip_handle* hd, DevOp& op, const double* xd, double* yd, in
     const int blocks = (op.p1.nwi + 3) / 4;
     hipLaunchKernelGGL((k_slab<T, T, 8, 1>), dim3(blocks), dim3(256), 0, st, op.p1.wis, op.p1.nwi,
                        op.p1.gidx, static_cast<const T*>(op.p1.data), xt, xt,
-                       static_cast<T*>(op.p1.part), nullptr, dof, op.p1.reds, op.p1.cnt);
+                       static_cast<T*>(op.p1.part), nullptr, dof);
   }
   if (hd->timing) hipEventRecord(hd->ev[2], st);
   if (op.p1.nbig > 0)
@@ -483,8 +604,7 @@ int launch_matvec(bw_hip_handle* hd, DevOp& op, const double* xd, double* yd, in
     const int blocks = (op.p2.nwi + 3) / 4;
     hipLaunchKernelGGL((k_slab<T, double, 8, 2>), dim3(blocks), dim3(256), 0, st, op.p2.wis,
                        op.p2.nwi, op.p2.gidx, static_cast<const T*>(op.p2.data), xt, yd,
-                       static_cast<T*>(op.p2.part), orig ? op.perm_out : nullptr, dof,
-                       op.p2.reds, op.p2.cnt);
+                       static_cast<T*>(op.p2.part), orig ? op.perm_out : nullptr, dof);
   }
   if (hd->timing) hipEventRecord(hd->ev[4], st);
   if (op.p2.nbig > 0)
@@ -672,12 +792,68 @@ int bw_hip_rmatvec(bw_hip_handle* hd, const double* x, double* y, int ordering, 
 int bw_hip_matvec_batched(bw_hip_handle* hd, int nrhs, const double* X, int64_t ldx, double* Y,
                           int64_t ldy, int ordering, int precision, void* stream) {
   if (!hd || !X || !Y || nrhs <= 0) return fail(BW_EINVAL, "matvec_batched: bad argument");
-  const DevOp& op = hd->op[0][precision == BW_PREC_FP32 ? 1 : 0];
+  if (ordering != BW_ORDER_PERMUTED && ordering != BW_ORDER_ORIGINAL)
+    return fail(BW_EINVAL, "matvec_batched: bad ordering");
+  if (precision != BW_PREC_FP64 && precision != BW_PREC_FP32)
+    return fail(BW_EINVAL, "matvec_batched: bad precision");
+  DevOp& op = hd->op[0][precision == BW_PREC_FP32 ? 1 : 0];
   if (!op.present) return fail(BW_ENOTSUP, "matvec_batched: precision not built");
   if (ldx < op.n_in || ldy < op.n_out) return fail(BW_ESHAPE, "matvec_batched: leading dimension too small");
-  for (int k = 0; k < nrhs; ++k) {
-    int rc = do_matvec(hd, false, X + k * ldx, Y + k * ldy, ordering, precision, 1, stream);
-    if (rc) return rc;
+  if (precision == BW_PREC_FP64) {
+    // FP64: the streaming path, one right-hand side at a time
+    for (int k = 0; k < nrhs; ++k) {
+      int rc = do_matvec(hd, false, X + k * ldx, Y + k * ldy, ordering, precision, 1, stream);
+      if (rc) return rc;
+    }
+    return BW_OK;
+  }
+  HIPCHK(hipSetDevice(hd->device));
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (!op.xb) {
+    HIPCHK(hipMalloc(&op.xb, static_cast<size_t>(op.xt_size) * kRhs * sizeof(float)));
+    hd->allocs.push_back(op.xb);
+    HIPCHK(hipMemset(op.xb, 0, static_cast<size_t>(op.xt_size) * kRhs * sizeof(float)));
+    for (int p = 0; p < 2; ++p) {
+      HIPCHK(hipMalloc(&op.partb[p], std::max<int64_t>(op.part_elems[p], 1) * kRhs * sizeof(float)));
+      hd->allocs.push_back(op.partb[p]);
+    }
+  }
+  const bool orig = ordering == BW_ORDER_ORIGINAL;
+  const int dof = hd->dof;
+  for (int k0 = 0; k0 < nrhs; k0 += kRhs) {
+    const int nk = std::min(kRhs, nrhs - k0);
+    const double* Xk = X + static_cast<int64_t>(k0) * ldx;
+    double* Yk = Y + static_cast<int64_t>(k0) * ldy;
+    {
+      const int64_t rows_per_block = 256 / kRhs;
+      const int blocks = static_cast<int>((op.n_in + rows_per_block - 1) / rows_per_block);
+      hipLaunchKernelGGL(k_gather_b, dim3(blocks), dim3(256), 0, st, Xk, ldx, nk, op.xb, op.n_in,
+                         orig ? op.perm_in : nullptr, dof);
+    }
+    if (op.p1.nwi > 0)
+      hipLaunchKernelGGL(k_slab_mfma<1>, dim3((op.p1.nwi + 3) / 4), dim3(256), 0, st, op.p1.wis,
+                         op.p1.nwi, op.p1.gidx, static_cast<const float*>(op.p1.data), op.xb, op.xb,
+                         nullptr, 0, kRhs, op.partb[0], nullptr, dof, op.p1.reds, op.p1.cnt);
+    if (op.p1.nred > 0)
+      hipLaunchKernelGGL(k_reduce_b<1>, dim3(op.p1.nred), dim3(64), 0, st, op.p1.reds,
+                         op.p1.nred, op.partb[0], op.xb, nullptr, 0, kRhs, nullptr, dof);
+    if (op.p1.nbig > 0)
+      hipLaunchKernelGGL(k_reduce_b<1>, dim3(op.p1.nbig), dim3(64), 0, st, op.p1.big_reds,
+                         op.p1.nbig, op.partb[0], op.xb, nullptr, 0, kRhs, nullptr, dof);
+    if (op.p2.nwi > 0)
+      hipLaunchKernelGGL(k_slab_mfma<2>, dim3((op.p2.nwi + 3) / 4), dim3(256), 0, st, op.p2.wis,
+                         op.p2.nwi, op.p2.gidx, static_cast<const float*>(op.p2.data), op.xb,
+                         nullptr, Yk, ldy, nk, op.partb[1], orig ? op.perm_out : nullptr, dof,
+                         op.p2.reds, op.p2.cnt);
+    if (op.p2.nred > 0)
+      hipLaunchKernelGGL(k_reduce_b<2>, dim3(op.p2.nred), dim3(64), 0, st, op.p2.reds,
+                         op.p2.nred, op.partb[1], nullptr, Yk, ldy, nk,
+                         orig ? op.perm_out : nullptr, dof);
+    if (op.p2.nbig > 0)
+      hipLaunchKernelGGL(k_reduce_b<2>, dim3(op.p2.nbig), dim3(64), 0, st, op.p2.big_reds,
+                         op.p2.nbig, op.partb[1], nullptr, Yk, ldy, nk,
+                         orig ? op.perm_out : nullptr, dof);
+    HIPCHK(hipGetLastError());
   }
   return BW_OK;
 }

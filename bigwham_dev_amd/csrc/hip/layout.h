@@ -52,9 +52,9 @@ struct WorkItem {
                      // partials (fused), or -1
 };
 
-// Reduce descriptor for a slab split over several WIs (16 bytes).  Slabs of
-// at most kFuseMaxParts WIs are summed by their last WI to finish (counted
-// with a device-side atomic); larger ones by a separate reduce pass.
+// Reduce descriptor for a slab split over several WIs (16 bytes).  The
+// partial sums are added by a separate reduce pass after the slab pass
+// (kFuseMaxParts = 0: no slab is reduced inside the slab kernel).
 struct Reduce {
   int32_t out_row;
   int32_t rows;
@@ -62,7 +62,7 @@ struct Reduce {
   int32_t nparts;
 };
 
-constexpr int kFuseMaxParts = 48;
+constexpr int kFuseMaxParts = 0;
 
 struct Piece {
   int64_t block;   // index into the view's FR or LR arrays

@@ -97,3 +97,23 @@ def test_batched(gpu, cases):
     Y = dev.matvec_batched(torch.from_numpy(X).to(gpu)).cpu().numpy()
     for k in range(8):
         assert rel(Y[:, k], O.matvec_original(h.flat(), X[:, k])) <= TOL64
+
+
+@pytest.mark.parametrize("nrhs", [5, 32, 40])
+@pytest.mark.parametrize("original", [False, True])
+def test_batched_fp32_mfma(gpu, cases, nrhs, original):
+    """Many right-hand sides on FP32 MFMA (v_mfma_f32_32x32x2f32) against the
+    oracle, column by column; nrhs below, at and above the 32-wide tile."""
+    import torch
+    for name in ("3DR0", "2DS0", "gaussian"):
+        h = cases[name]
+        print(f"case {name} nrhs={nrhs} original={original}", flush=True)
+        dev = HipHmat(h, precisions=("fp64", "fp32"))
+        rng = np.random.default_rng(nrhs)
+        X = rng.uniform(-1, 1, (h.size, nrhs))
+        Y = dev.matvec_batched(torch.from_numpy(X).to(gpu), original=original,
+                               precision="fp32").cpu().numpy()
+        f = h.flat()
+        for k in range(nrhs):
+            ref = (O.matvec_original if original else O.matvec)(f, X[:, k])
+            assert rel(Y[:, k], ref) <= TOL32, (name, k)

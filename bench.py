@@ -98,6 +98,8 @@ def main():
     ap.add_argument("--gmres", action="store_true", help="also time GMRES to 1e-8")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--wi-bytes", type=int, default=0)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default unroll depth instead of timing 4/8/16 first")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only; multi-GPU sharding is a design "
@@ -127,6 +129,13 @@ def main():
     torch.cuda.synchronize()
     t_upload = time.perf_counter() - t0
     log(f"layout + upload {t_upload:.1f}s, device bytes {dev.device_bytes / 1e9:.2f} GB")
+    tuning = None
+    if not args.no_autotune:
+        from bigwham_dev_amd import autotune
+        t0 = time.perf_counter()
+        tr = autotune.tune_unroll(dev, use_cache=False)
+        tuning = {"unroll": tr.unroll, "ms": tr.ms, "seconds": round(time.perf_counter() - t0, 2)}
+        log(f"autotune {tuning}")
 
     rng = np.random.default_rng(1234)
     x_np = rng.uniform(-1.0, 1.0, n)
@@ -192,6 +201,7 @@ def main():
         "fp32": {"value": round(args.steps / dt32, 3), "unit": "matvec/s",
                  "ms_per_step": round(dt32 / args.steps * 1e3, 4), "roofline": rf32},
         "setup_s": {"host_build": round(t_build, 2), "layout_upload": round(t_upload, 2)},
+        "tuning": tuning,
     }
 
     if not args.no_cpu:

@@ -1,0 +1,125 @@
+"""Launch-schedule autotuner for the slab kernels.
+
+Two knobs shape a matvec's schedule on the device:
+
+- the unroll depth of ``k_slab`` (loads in flight per lane: 4, 8 or 16),
+  switchable per handle at no cost (``bw_hip_set_unroll``);
+- the work-item size (bytes of factors one workgroup streams), fixed when the
+  layout is built (``bw_hip_set_tuning`` before ``HipHmat``).
+
+``tune_unroll`` times each unroll depth on the handle's own data with the
+engine's HIP-event timers and keeps the fastest one.  ``tune_layout`` rebuilds
+the handle for each candidate work-item size (a full re-layout and upload, so
+only worth it for matrices that are applied many times) and returns the best.
+Results are cached per matrix signature in ``~/.cache/bigwham_dev_amd/tune.json``
+(``BIGWHAM_TUNE_CACHE`` overrides the path) so a later run of the same problem
+skips the search.
+"""
+from __future__ import annotations
+
+import json
+import os
+from dataclasses import dataclass, field
+
+UNROLLS = (4, 8, 16)
+WI_BYTES = (48 << 10, 96 << 10, 192 << 10)
+
+
+@dataclass
+class TuneResult:
+    unroll: dict = field(default_factory=dict)      # precision -> best unroll
+    ms: dict = field(default_factory=dict)          # precision -> {unroll: ms}
+    wi_bytes: int = 0
+
+
+def _cache_path() -> str:
+    return os.environ.get("BIGWHAM_TUNE_CACHE",
+                          os.path.join(os.path.expanduser("~"), ".cache", "bigwham_dev_amd",
+                                       "tune.json"))
+
+
+def signature(dev) -> str:
+    """Key of a matrix for the cache: size, entries and device bytes."""
+    return f"n{dev.n_rows}x{dev.n_cols}-e{dev.nb_entries}-b{dev.device_bytes}"
+
+
+def _load_cache() -> dict:
+    try:
+        with open(_cache_path()) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _store_cache(key: str, value: dict) -> None:
+    path = _cache_path()
+    try:
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        data = _load_cache()
+        data[key] = value
+        with open(path, "w") as f:
+            json.dump(data, f, indent=1)
+    except OSError:
+        pass
+
+
+def time_matvec(dev, x, y, precision: str, reps: int) -> float:
+    """Average device time (ms) of one matvec over ``reps`` calls."""
+    import torch
+    dev.matvec(x, original=True, precision=precision, out=y)
+    dev.set_timing(True)
+    for _ in range(reps):
+        dev.matvec(x, original=True, precision=precision, out=y)
+    torch.cuda.synchronize()
+    st = dev.timing_stats()
+    dev.set_timing(False)
+    return float(st["total"])
+
+
+def tune_unroll(dev, precisions=None, reps: int = 5, use_cache: bool = True) -> TuneResult:
+    """Pick the fastest unroll depth per precision and apply it to ``dev``."""
+    import torch
+    precisions = tuple(precisions or dev.precisions)
+    key = "unroll:" + signature(dev)
+    res = TuneResult()
+    cached = _load_cache().get(key) if use_cache else None
+    if cached and all(p in cached for p in precisions):
+        res.unroll = {p: int(cached[p]) for p in precisions}
+    else:
+        x = torch.ones(dev.n_cols, dtype=torch.float64, device=dev.torch_device)
+        y = torch.empty(dev.n_rows, dtype=torch.float64, device=dev.torch_device)
+        for p in precisions:
+            res.ms[p] = {}
+            for u in UNROLLS:
+                dev.set_unroll(**{p: u})
+                res.ms[p][u] = time_matvec(dev, x, y, p, reps)
+            res.unroll[p] = min(res.ms[p], key=res.ms[p].get)
+        if use_cache:
+            _store_cache(key, dict(res.unroll))
+    for p, u in res.unroll.items():
+        dev.set_unroll(**{p: u})
+    return res
+
+
+def tune_layout(make_dev, precisions=("fp64", "fp32"), candidates=WI_BYTES, reps: int = 5):
+    """Build the handle once per work-item size (``make_dev()`` after
+    ``set_tuning``), keep the fastest; returns (dev, TuneResult)."""
+    import torch
+    from .hmat import set_tuning
+    best = None
+    for wi in candidates:
+        set_tuning(wi, wi)
+        dev = make_dev()
+        r = tune_unroll(dev, precisions, reps, use_cache=False)
+        x = torch.ones(dev.n_cols, dtype=torch.float64, device=dev.torch_device)
+        y = torch.empty(dev.n_rows, dtype=torch.float64, device=dev.torch_device)
+        score = sum(time_matvec(dev, x, y, p, reps) for p in precisions)
+        if best is None or score < best[0]:
+            if best is not None:
+                best[1].close()
+            r.wi_bytes = wi
+            best = (score, dev, r)
+        else:
+            dev.close()
+    set_tuning(0, 0)
+    return best[1], best[2]

@@ -374,6 +374,7 @@ struct DevOp {
   const int64_t* perm_out = nullptr;  // device, n_out / dof
   int64_t stream_bytes = 0;
   int64_t part_elems[2] = {0, 0};
+  int unroll = 8;             // loads in flight per lane in k_slab (4, 8 or 16)
   // many right-hand sides (FP32 MFMA path), allocated on first use
   float* xb = nullptr;        // xt_size x kRhs
   float* partb[2] = {nullptr, nullptr};
@@ -575,6 +576,21 @@ void compute_diagonal(bw_hip_handle* hd, const bw_flat_hmat* h) {
     for (int64_t j = 0; j < d; ++j) hd->diag_orig[d * h->perm1[i] + j] = hd->diag_perm[d * i + j];
 }
 
+template <typename T, typename OutT, int PASS>
+void launch_slab(int unroll, int blocks, hipStream_t st, const bwl::WorkItem* wis, int nwi,
+                 const int32_t* gidx, const T* data, const T* xin, OutT* out, T* part,
+                 const int64_t* perm, int dof) {
+  if (unroll == 4)
+    hipLaunchKernelGGL((k_slab<T, OutT, 4, PASS>), dim3(blocks), dim3(256), 0, st, wis, nwi, gidx,
+                       data, xin, out, part, perm, dof);
+  else if (unroll == 16)
+    hipLaunchKernelGGL((k_slab<T, OutT, 16, PASS>), dim3(blocks), dim3(256), 0, st, wis, nwi, gidx,
+                       data, xin, out, part, perm, dof);
+  else
+    hipLaunchKernelGGL((k_slab<T, OutT, 8, PASS>), dim3(blocks), dim3(256), 0, st, wis, nwi, gidx,
+                       data, xin, out, part, perm, dof);
+}
+
 template <typename T>
 int launch_matvec(bw_hip_handle* hd, DevOp& op, const double* xd, double* yd, int ordering,
                   hipStream_t st) {
@@ -591,9 +607,9 @@ int launch_matvec(bw_hip_handle* hd, DevOp& op, const double* xd, double* yd, in
   if (hd->timing) hipEventRecord(hd->ev[1], st);
   if (op.p1.nwi > 0) {
     const int blocks = (op.p1.nwi + 3) / 4;
-    hipLaunchKernelGGL((k_slab<T, T, 8, 1>), dim3(blocks), dim3(256), 0, st, op.p1.wis, op.p1.nwi,
-                       op.p1.gidx, static_cast<const T*>(op.p1.data), xt, xt,
-                       static_cast<T*>(op.p1.part), nullptr, dof);
+    launch_slab<T, T, 1>(op.unroll, blocks, st, op.p1.wis, op.p1.nwi, op.p1.gidx,
+                         static_cast<const T*>(op.p1.data), xt, xt, static_cast<T*>(op.p1.part),
+                         nullptr, dof);
   }
   if (hd->timing) hipEventRecord(hd->ev[2], st);
   if (op.p1.nbig > 0)
@@ -602,9 +618,9 @@ int launch_matvec(bw_hip_handle* hd, DevOp& op, const double* xd, double* yd, in
   if (hd->timing) hipEventRecord(hd->ev[3], st);
   if (op.p2.nwi > 0) {
     const int blocks = (op.p2.nwi + 3) / 4;
-    hipLaunchKernelGGL((k_slab<T, double, 8, 2>), dim3(blocks), dim3(256), 0, st, op.p2.wis,
-                       op.p2.nwi, op.p2.gidx, static_cast<const T*>(op.p2.data), xt, yd,
-                       static_cast<T*>(op.p2.part), orig ? op.perm_out : nullptr, dof);
+    launch_slab<T, double, 2>(op.unroll, blocks, st, op.p2.wis, op.p2.nwi, op.p2.gidx,
+                              static_cast<const T*>(op.p2.data), xt, yd,
+                              static_cast<T*>(op.p2.part), orig ? op.perm_out : nullptr, dof);
   }
   if (hd->timing) hipEventRecord(hd->ev[4], st);
   if (op.p2.nbig > 0)
@@ -876,6 +892,18 @@ int bw_hip_sizes(bw_hip_handle* hd, int64_t* n_rows, int64_t* n_cols, int64_t* n
   if (device_bytes) *device_bytes = hd->device_bytes;
   if (sb64) *sb64 = hd->op[0][0].present ? hd->op[0][0].stream_bytes : 0;
   if (sb32) *sb32 = hd->op[0][1].present ? hd->op[0][1].stream_bytes : 0;
+  return BW_OK;
+}
+
+int bw_hip_set_unroll(bw_hip_handle* hd, int unroll_fp64, int unroll_fp32) {
+  if (!hd) return fail(BW_EINVAL, "set_unroll: null handle");
+  for (int u : {unroll_fp64, unroll_fp32})
+    if (u != 0 && u != 4 && u != 8 && u != 16)
+      return fail(BW_EINVAL, "set_unroll: unroll must be 4, 8 or 16 (0 keeps the current one)");
+  for (int adj = 0; adj < 2; ++adj) {
+    if (unroll_fp64) hd->op[adj][0].unroll = unroll_fp64;
+    if (unroll_fp32) hd->op[adj][1].unroll = unroll_fp32;
+  }
   return BW_OK;
 }
 

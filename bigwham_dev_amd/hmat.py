@@ -198,12 +198,18 @@ class HipHmat:
         self.nb_entries = e.value
         self.device_bytes = db.value
         self.stream_bytes = {"fp64": s64.value, "fp32": s32.value}
+        self.device = int(device)
+        self.torch_device = f"cuda:{device}" if device >= 0 else "cuda"
+
+    def close(self) -> None:
+        """Free the device copy now (also done when the object is collected)."""
+        if getattr(self, "_h", None):
+            N.hip().bw_hip_destroy(self._h)
+            self._h = None
 
     def __del__(self):
         try:
-            if self._h:
-                N.hip().bw_hip_destroy(self._h)
-                self._h = None
+            self.close()
         except Exception:
             pass
 
@@ -262,6 +268,9 @@ class HipHmat:
         N.check_hip(N.hip().bw_hip_diagonal(self._h, N.dptr(d),
                                             N.BW_ORDER_ORIGINAL if original else N.BW_ORDER_PERMUTED))
         return d
+
+    def set_unroll(self, fp64: int = 0, fp32: int = 0) -> None:
+        N.check_hip(N.hip().bw_hip_set_unroll(self._h, int(fp64), int(fp32)))
 
     def set_timing(self, enable: bool = True) -> None:
         N.check_hip(N.hip().bw_hip_set_timing(self._h, int(enable)))

@@ -83,6 +83,10 @@ int bw_hip_sizes(bw_hip_handle* hd, int64_t* n_rows, int64_t* n_cols,
 /* Tuning knobs for the next create (work-item target bytes; 0 = default). */
 int bw_hip_set_tuning(int64_t wi_bytes_fp64, int64_t wi_bytes_fp32);
 
+/* Loads in flight per lane in the slab kernels of this handle (4, 8 or 16;
+ * 0 keeps the current value).  Chosen by bigwham_dev_amd.autotune. */
+int bw_hip_set_unroll(bw_hip_handle* hd, int unroll_fp64, int unroll_fp32);
+
 /* Per-kernel timing.  With timing enabled every matvec records HIP events
  * between its kernels on the call's stream; bw_hip_timing_stats returns the
  * average milliseconds over the calls since bw_hip_set_timing (which resets

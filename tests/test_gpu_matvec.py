@@ -117,3 +117,29 @@ def test_batched_fp32_mfma(gpu, cases, nrhs, original):
         for k in range(nrhs):
             ref = (O.matvec_original if original else O.matvec)(f, X[:, k])
             assert rel(Y[:, k], ref) <= TOL32, (name, k)
+
+
+@pytest.mark.parametrize("unroll", [4, 8, 16])
+def test_unroll_depths(gpu, cases, unroll):
+    """Every unroll depth the autotuner may pick keeps parity."""
+    import torch
+    h = cases["3DR0"]
+    print(f"unroll {unroll} n={h.size}", flush=True)
+    dev = HipHmat(h, precisions=("fp64", "fp32"))
+    dev.set_unroll(fp64=unroll, fp32=unroll)
+    x = np.random.default_rng(11).uniform(-1, 1, h.size)
+    y_ref = O.matvec_original(h.flat(), x)
+    xd = torch.from_numpy(x).to(gpu)
+    assert rel(dev.matvec(xd).cpu().numpy(), y_ref) <= TOL64
+    assert rel(dev.matvec(xd, precision="fp32").cpu().numpy(), y_ref) <= TOL32
+
+
+def test_autotune_unroll(gpu, cases, tmp_path, monkeypatch):
+    from bigwham_dev_amd import autotune
+    monkeypatch.setenv("BIGWHAM_TUNE_CACHE", str(tmp_path / "tune.json"))
+    dev = HipHmat(cases["3DR0"], precisions=("fp64", "fp32"))
+    r = autotune.tune_unroll(dev, reps=2)
+    assert set(r.unroll) == {"fp64", "fp32"}
+    assert all(u in autotune.UNROLLS for u in r.unroll.values())
+    r2 = autotune.tune_unroll(dev, reps=2)  # from the cache
+    assert r2.unroll == r.unroll and not r2.ms

@@ -14,6 +14,7 @@ import numpy as np
 LIB_DIR = Path(__file__).resolve().parent / "lib"
 
 BW_OK = 0
+BW_EINVAL, BW_ENOMEM, BW_EHIP, BW_ESHAPE, BW_ESTATE, BW_ENOTSUP = -1, -2, -3, -4, -5, -6
 BW_PREC_FP64 = 1
 BW_PREC_FP32 = 2
 BW_FLAG_ADJOINT = 1
@@ -113,6 +114,22 @@ def hip() -> C.CDLL:
         lib.bw_layout_free.argtypes = [_vp]
         lib.bw_layout_free.restype = None
         lib.bw_layout_last_error.restype = C.c_char_p
+        lib.bw_layout_build_multi.argtypes = [C.POINTER(C.POINTER(FlatHmat)), C.c_int, C.c_int,
+                                              C.c_int, C.c_int64, C.POINTER(_vp)]
+        lib.bw_hip_create_multi.argtypes = [C.POINTER(C.POINTER(FlatHmat)), C.c_int, C.c_int,
+                                            C.c_int, C.c_int, C.POINTER(_vp), _i64, _i64]
+        lib.bw_hip_graph_create.argtypes = [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int,
+                                            C.POINTER(_vp)]
+        lib.bw_hip_graph_launch.argtypes = [_vp, _vp]
+        lib.bw_hip_graph_destroy.argtypes = [_vp]
+        lib.bw_hip_graph_destroy.restype = None
+        lib.bw_flat_compose.argtypes = [C.POINTER(C.POINTER(FlatHmat)), C.c_int, C.POINTER(_vp)]
+        lib.bw_flat_composite_view.argtypes = [_vp]
+        lib.bw_flat_composite_view.restype = C.POINTER(FlatHmat)
+        lib.bw_flat_composite_offsets.argtypes = [_vp, _i64, _i64]
+        lib.bw_flat_compose_last_error.restype = C.c_char_p
+        lib.bw_flat_composite_free.argtypes = [_vp]
+        lib.bw_flat_composite_free.restype = None
         _hip = lib
     return _hip
 
@@ -141,7 +158,11 @@ HOST_SYMBOLS = ["bwh_create", "bwh_create_gaussian", "bwh_build_pattern", "bwh_b
                 "bwh_rotation_matrices", "bwh_dense_block", "bwh_max_aca_error",
                 "bwh_timings", "bwh_free", "bwh_last_error", "bwh_influence"]
 HIP_SYMBOLS = ["bw_hip_create", "bw_hip_matvec", "bw_hip_rmatvec", "bw_hip_matvec_batched",
-               "bw_hip_diagonal", "bw_hip_sizes", "bw_hip_set_tuning", "bw_hip_set_unroll", "bw_hip_set_timing",
+               "bw_hip_diagonal", "bw_hip_sizes", "bw_hip_set_tuning", "bw_hip_set_unroll",
+               "bw_hip_set_timing",
                "bw_hip_timing_stats", "bw_hip_destroy", "bw_hip_last_error",
                "bw_layout_build", "bw_layout_info", "bw_layout_export", "bw_layout_free",
-               "bw_layout_last_error"]
+               "bw_layout_last_error", "bw_layout_build_multi", "bw_hip_create_multi",
+               "bw_hip_graph_create", "bw_hip_graph_launch", "bw_hip_graph_destroy",
+               "bw_flat_compose", "bw_flat_composite_view", "bw_flat_composite_offsets",
+               "bw_flat_compose_last_error", "bw_flat_composite_free"]

@@ -713,12 +713,20 @@ void bw_hip_destroy(bw_hip_handle* hd) {
   delete hd;
 }
 
-int bw_hip_create(const bw_flat_hmat* h, int precision_mask, int flags, int device,
-                  bw_hip_handle** out) {
+}  // extern "C"
+
+namespace bwl {
+int compose_flat(const bw_flat_hmat* const* hs, int n, bw_flat_composite* c, std::string* msg);
+}
+
+namespace {
+
+int create_impl(const bw_flat_hmat* h, long double expected_area, int precision_mask, int flags,
+                int device, bw_hip_handle** out) {
   if (!out) return fail(BW_EINVAL, "create: null output pointer");
   *out = nullptr;
   std::string msg;
-  int rc = bwl::validate_flat(h, &msg);
+  int rc = bwl::validate_flat(h, &msg, expected_area);
   if (rc) return fail(rc, "create: " + msg);
   if ((precision_mask & (BW_PREC_FP64 | BW_PREC_FP32)) == 0 ||
       (precision_mask & ~(BW_PREC_FP64 | BW_PREC_FP32)))
@@ -793,6 +801,99 @@ int bw_hip_create(const bw_flat_hmat* h, int precision_mask, int flags, int devi
       if (hipEventCreate(&e) != hipSuccess) return cleanup(fail(BW_EHIP, "create: hipEventCreate"));
   *out = hd;
   return BW_OK;
+}
+
+}  // namespace
+
+struct bw_hip_graph {
+  int device = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+extern "C" {
+
+int bw_hip_create(const bw_flat_hmat* h, int precision_mask, int flags, int device,
+                  bw_hip_handle** out) {
+  return create_impl(h, -1.0L, precision_mask, flags, device, out);
+}
+
+int bw_hip_create_multi(const bw_flat_hmat* const* hs, int n_mats, int precision_mask, int flags,
+                        int device, bw_hip_handle** out, int64_t* row_off, int64_t* col_off) {
+  if (!out) return fail(BW_EINVAL, "create_multi: null output pointer");
+  *out = nullptr;
+  bw_flat_composite* c = nullptr;
+  int rc = bw_flat_compose(hs, n_mats, &c);
+  if (rc) return fail(rc, std::string("create_multi: ") + bw_flat_compose_last_error());
+  const bw_flat_hmat* f = bw_flat_composite_view(c);
+  long double area = 0.0L;
+  for (int m = 0; m < n_mats; ++m)
+    area += static_cast<long double>(hs[m]->n_row_pts) * hs[m]->n_col_pts;
+  rc = create_impl(f, area, precision_mask, flags, device, out);
+  if (!rc) bw_flat_composite_offsets(c, row_off, col_off);
+  bw_flat_composite_free(c);
+  return rc;
+}
+
+int bw_hip_graph_create(bw_hip_handle* hd, const double* x, double* y, int ordering,
+                        int precision, int adjoint, bw_hip_graph** out) {
+  if (!hd || !x || !y || !out) return fail(BW_EINVAL, "graph_create: null argument");
+  *out = nullptr;
+  if (ordering != BW_ORDER_PERMUTED && ordering != BW_ORDER_ORIGINAL)
+    return fail(BW_EINVAL, "graph_create: ordering must be 0 (permuted) or 1 (original)");
+  if (precision != BW_PREC_FP64 && precision != BW_PREC_FP32)
+    return fail(BW_EINVAL, "graph_create: precision must be BW_PREC_FP64 or BW_PREC_FP32");
+  DevOp& op = hd->op[adjoint ? 1 : 0][precision == BW_PREC_FP32 ? 1 : 0];
+  if (!op.present) return fail(BW_ENOTSUP, "graph_create: adjoint / precision not built");
+  HIPCHK(hipSetDevice(hd->device));
+  hipStream_t cs;
+  HIPCHK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+  const bool timing = hd->timing;
+  hd->timing = false;  // no event records inside the graph
+  hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+  int rc = BW_OK;
+  hipGraph_t graph = nullptr;
+  if (e == hipSuccess) {
+    rc = precision == BW_PREC_FP64 ? launch_matvec<double>(hd, op, x, y, ordering, cs)
+                                   : launch_matvec<float>(hd, op, x, y, ordering, cs);
+    hipError_t e2 = hipStreamEndCapture(cs, &graph);
+    if (!rc && e2 != hipSuccess) rc = fail(BW_EHIP, std::string("graph_create: ") + hipGetErrorString(e2));
+  } else {
+    rc = fail(BW_EHIP, std::string("graph_create: ") + hipGetErrorString(e));
+  }
+  hd->timing = timing;
+  hipStreamDestroy(cs);
+  if (rc) {
+    if (graph) hipGraphDestroy(graph);
+    return rc;
+  }
+  auto* g = new bw_hip_graph();
+  g->device = hd->device;
+  g->graph = graph;
+  e = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    hipGraphDestroy(graph);
+    delete g;
+    return fail(BW_EHIP, std::string("graph_create: instantiate: ") + hipGetErrorString(e));
+  }
+  *out = g;
+  return BW_OK;
+}
+
+int bw_hip_graph_launch(bw_hip_graph* g, void* stream) {
+  if (!g) return fail(BW_EINVAL, "graph_launch: null graph");
+  HIPCHK(hipSetDevice(g->device));
+  HIPCHK(hipGraphLaunch(g->exec, static_cast<hipStream_t>(stream)));
+  return BW_OK;
+}
+
+void bw_hip_graph_destroy(bw_hip_graph* g) {
+  if (!g) return;
+  hipSetDevice(g->device);
+  hipDeviceSynchronize();
+  if (g->exec) hipGraphExecDestroy(g->exec);
+  if (g->graph) hipGraphDestroy(g->graph);
+  delete g;
 }
 
 int bw_hip_matvec(bw_hip_handle* hd, const double* x, double* y, int ordering, int precision,

@@ -19,7 +19,7 @@ int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
 
 }  // namespace
 
-int validate_flat(const bw_flat_hmat* h, std::string* msg) {
+int validate_flat(const bw_flat_hmat* h, std::string* msg, long double expected_area) {
   if (!h) {
     *msg = "null bw_flat_hmat";
     return BW_EINVAL;
@@ -76,7 +76,8 @@ int validate_flat(const bw_flat_hmat* h, std::string* msg) {
     }
     area += static_cast<long double>(i1 - i0) * (j1 - j0);
   }
-  if (area != static_cast<long double>(h->n_row_pts) * h->n_col_pts) {
+  if (expected_area < 0.0L) expected_area = static_cast<long double>(h->n_row_pts) * h->n_col_pts;
+  if (area != expected_area) {
     *msg = "blocks do not tile the matrix";
     return BW_ESHAPE;
   }

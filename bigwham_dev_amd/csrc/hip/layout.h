@@ -139,7 +139,9 @@ struct Layout {
 };
 
 // Validates the flat H-matrix; returns BW_OK or an error code with message.
-int validate_flat(const bw_flat_hmat* h, std::string* msg);
+// The blocks must cover `expected_area` point pairs: the whole matrix by
+// default, the diagonal blocks' total for a block-diagonal composite.
+int validate_flat(const bw_flat_hmat* h, std::string* msg, long double expected_area = -1.0L);
 
 void make_view(const bw_flat_hmat* h, bool transposed, OpView* v);
 void build_layout(const OpView& v, const LayoutOptions& opt, Layout* L);

@@ -13,17 +13,21 @@ thread_local std::string g_lerr;
 
 struct bw_layout {
   bwl::Layout L;
+  bw_flat_composite* composite = nullptr;  // owned when built by bw_layout_build_multi
+  ~bw_layout() { bw_flat_composite_free(composite); }
 };
 
 extern "C" {
 
 const char* bw_layout_last_error(void) { return g_lerr.c_str(); }
 
-int bw_layout_build(const bw_flat_hmat* h, int precision, int adjoint, int64_t wi_bytes,
-                    bw_layout** out) {
+namespace {
+
+int build_checked(const bw_flat_hmat* h, long double area, int precision, int adjoint,
+                  int64_t wi_bytes, bw_layout** out) {
   if (!out) return BW_EINVAL;
   std::string msg;
-  int rc = bwl::validate_flat(h, &msg);
+  int rc = bwl::validate_flat(h, &msg, area);
   if (rc) {
     g_lerr = msg;
     return rc;
@@ -42,6 +46,34 @@ int bw_layout_build(const bw_flat_hmat* h, int precision, int adjoint, int64_t w
   bwl::build_layout(v, opt, &lay->L);
   *out = lay;
   return BW_OK;
+}
+
+}  // namespace
+
+int bw_layout_build(const bw_flat_hmat* h, int precision, int adjoint, int64_t wi_bytes,
+                    bw_layout** out) {
+  return build_checked(h, -1.0L, precision, adjoint, wi_bytes, out);
+}
+
+int bw_layout_build_multi(const bw_flat_hmat* const* hs, int n_mats, int precision, int adjoint,
+                          int64_t wi_bytes, bw_layout** out) {
+  if (!out) return BW_EINVAL;
+  *out = nullptr;
+  bw_flat_composite* c = nullptr;
+  int rc = bw_flat_compose(hs, n_mats, &c);
+  if (rc) {
+    g_lerr = bw_flat_compose_last_error();
+    return rc;
+  }
+  long double area = 0.0L;
+  for (int m = 0; m < n_mats; ++m)
+    area += static_cast<long double>(hs[m]->n_row_pts) * hs[m]->n_col_pts;
+  // the layout's view points into the composite, which the layout then owns;
+  // block payloads stay with the inputs, which must outlive the layout
+  rc = build_checked(bw_flat_composite_view(c), area, precision, adjoint, wi_bytes, out);
+  if (!rc) (*out)->composite = c;
+  else bw_flat_composite_free(c);
+  return rc;
 }
 
 // info[0..1] = xt_t, xt_size; info[2+6p .. 7+6p] for pass p (0, 1) =

@@ -180,14 +180,40 @@ class HipHmat:
 
     def __init__(self, host: HostHmat, precisions=("fp64",), adjoint: bool = False,
                  device: int = -1):
-        mask = 0
-        for p in precisions:
-            mask |= {"fp64": N.BW_PREC_FP64, "fp32": N.BW_PREC_FP32}[p]
-        flags = N.BW_FLAG_ADJOINT if adjoint else 0
+        mask, flags = self._mask(precisions, adjoint)
         out = C.c_void_p()
         self._host = host  # host arrays may be freed after create; kept for convenience
         N.check_hip(N.hip().bw_hip_create(C.byref(host.flat()), mask, flags, int(device),
                                           C.byref(out)))
+        self._finish(out, precisions, adjoint, device)
+
+    @classmethod
+    def many(cls, hosts, precisions=("fp64",), adjoint: bool = False, device: int = -1):
+        """One operator for many independent meshes: diag(H_0, ..., H_{n-1}) on
+        the concatenated vectors (bw_hip_create_multi).  ``row_offsets`` /
+        ``col_offsets`` give each mesh's slice of y and x."""
+        self = cls.__new__(cls)
+        mask, flags = cls._mask(precisions, adjoint)
+        flats = [h.flat() for h in hosts]
+        arr = (C.POINTER(N.FlatHmat) * len(flats))(*[C.pointer(f) for f in flats])
+        ro = np.zeros(len(flats) + 1, np.int64)
+        co = np.zeros(len(flats) + 1, np.int64)
+        out = C.c_void_p()
+        N.check_hip(N.hip().bw_hip_create_multi(arr, len(flats), mask, flags, int(device),
+                                                C.byref(out), N.iptr(ro), N.iptr(co)))
+        self._host = list(hosts)
+        self._finish(out, precisions, adjoint, device)
+        self.row_offsets, self.col_offsets = ro, co
+        return self
+
+    @staticmethod
+    def _mask(precisions, adjoint):
+        mask = 0
+        for p in precisions:
+            mask |= {"fp64": N.BW_PREC_FP64, "fp32": N.BW_PREC_FP32}[p]
+        return mask, (N.BW_FLAG_ADJOINT if adjoint else 0)
+
+    def _finish(self, out, precisions, adjoint, device):
         self._h = out
         self.precisions = tuple(precisions)
         self.adjoint = adjoint
@@ -269,6 +295,11 @@ class HipHmat:
                                             N.BW_ORDER_ORIGINAL if original else N.BW_ORDER_PERMUTED))
         return d
 
+    def graph(self, x, y=None, original: bool = True, precision: str = "fp64",
+              adjoint: bool = False) -> "MatvecGraph":
+        """Capture y = H x (or H^T x) on these device tensors as one HIP graph."""
+        return MatvecGraph(self, x, y, original, precision, adjoint)
+
     def set_unroll(self, fp64: int = 0, fp32: int = 0) -> None:
         N.check_hip(N.hip().bw_hip_set_unroll(self._h, int(fp64), int(fp32)))
 
@@ -285,6 +316,46 @@ class HipHmat:
         out = dict(zip(keys, avg.tolist()))
         out["count"] = n.value
         return out
+
+
+class MatvecGraph:
+    """A matvec on fixed device vectors, replayed with one graph launch
+    (bw_hip_graph_create / bw_hip_graph_launch).  Shares the handle's
+    scratch: its launches must not overlap other calls on the same handle."""
+
+    def __init__(self, dev: HipHmat, x, y, original, precision, adjoint):
+        import torch
+        if not (isinstance(x, torch.Tensor) and x.is_cuda and x.dtype == torch.float64
+                and x.is_contiguous()):
+            raise TypeError("x must be a contiguous float64 CUDA tensor")
+        n_out = dev.n_cols if adjoint else dev.n_rows
+        if y is None:
+            y = torch.empty(n_out, dtype=torch.float64, device=x.device)
+        self.dev, self.x, self.y = dev, x, y
+        out = C.c_void_p()
+        N.check_hip(N.hip().bw_hip_graph_create(
+            dev._h, C.c_void_p(x.data_ptr()), C.c_void_p(y.data_ptr()),
+            N.BW_ORDER_ORIGINAL if original else N.BW_ORDER_PERMUTED,
+            HipHmat._prec(precision), int(adjoint), C.byref(out)))
+        self._g = out
+
+    def launch(self, stream: int | None = None):
+        import torch
+        if stream is None:
+            stream = torch.cuda.current_stream(self.x.device).cuda_stream
+        N.check_hip(N.hip().bw_hip_graph_launch(self._g, C.c_void_p(stream)))
+        return self.y
+
+    def close(self) -> None:
+        if getattr(self, "_g", None):
+            N.hip().bw_hip_graph_destroy(self._g)
+            self._g = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def set_tuning(wi_bytes_fp64: int = 0, wi_bytes_fp32: int = 0) -> None:

@@ -52,6 +52,16 @@ typedef struct bw_hip_handle bw_hip_handle;
 int bw_hip_create(const bw_flat_hmat* h, int precision_mask, int flags, int device,
                   bw_hip_handle** out);
 
+/* Many independent meshes in one operator: lays out the block-diagonal
+ * matrix diag(H_0, ..., H_{n_mats-1}) (same dof for all) so one matvec
+ * streams every mesh's blocks in the same five launches.  x and y are the
+ * concatenations of the per-mesh vectors; row_off / col_off (n_mats + 1
+ * int64 each, may be NULL) receive the DoF offsets of each mesh in y and x.
+ * No counterpart in the reference, which builds one Hmat per mesh. */
+int bw_hip_create_multi(const bw_flat_hmat* const* hs, int n_mats, int precision_mask,
+                        int flags, int device, bw_hip_handle** out, int64_t* row_off,
+                        int64_t* col_off);
+
 /* y = H x.  precision: BW_PREC_FP64 (FP64 factors, FP64 accumulation) or
  * BW_PREC_FP32 (FP32 factors, FP32 accumulation; x, y stay double).
  * on_device: 1 if x and y are device pointers, 0 for host pointers.
@@ -80,6 +90,26 @@ int bw_hip_sizes(bw_hip_handle* hd, int64_t* n_rows, int64_t* n_cols,
                  int64_t* nb_entries, int64_t* device_bytes,
                  int64_t* stream_bytes_fp64, int64_t* stream_bytes_fp32);
 
+/* A matvec on fixed device vectors x, y captured once as a HIP graph and
+ * replayed with one launch (no counterpart in the reference).  The graph uses
+ * the handle's scratch, so its launches must not overlap other calls on the
+ * handle; destroy it before the handle. */
+typedef struct bw_hip_graph bw_hip_graph;
+int bw_hip_graph_create(bw_hip_handle* hd, const double* x, double* y, int ordering,
+                        int precision, int adjoint, bw_hip_graph** out);
+int bw_hip_graph_launch(bw_hip_graph* g, void* stream);
+void bw_hip_graph_destroy(bw_hip_graph* g);
+
+/* Host-only block-diagonal composition (what bw_hip_create_multi lays out),
+ * for inspection and CPU checks.  The view borrows the inputs' block
+ * pointers; offsets are in DoF, n_mats + 1 entries. */
+typedef struct bw_flat_composite bw_flat_composite;
+int bw_flat_compose(const bw_flat_hmat* const* hs, int n_mats, bw_flat_composite** out);
+const bw_flat_hmat* bw_flat_composite_view(const bw_flat_composite* c);
+int bw_flat_composite_offsets(const bw_flat_composite* c, int64_t* row_off, int64_t* col_off);
+const char* bw_flat_compose_last_error(void);
+void bw_flat_composite_free(bw_flat_composite* c);
+
 /* Tuning knobs for the next create (work-item target bytes; 0 = default). */
 int bw_hip_set_tuning(int64_t wi_bytes_fp64, int64_t wi_bytes_fp32);
 
@@ -105,6 +135,10 @@ const char* bw_hip_last_error(void);
 typedef struct bw_layout bw_layout;
 int bw_layout_build(const bw_flat_hmat* h, int precision, int adjoint, int64_t wi_bytes,
                     bw_layout** out);
+/* Layout of the block-diagonal composite of bw_hip_create_multi; the inputs'
+ * blocks must outlive the layout. */
+int bw_layout_build_multi(const bw_flat_hmat* const* hs, int n_mats, int precision,
+                          int adjoint, int64_t wi_bytes, bw_layout** out);
 int bw_layout_info(const bw_layout* lay, int64_t* info /* 16 */);
 int bw_layout_export(const bw_layout* lay, int pass, int64_t* wis, int32_t* gidx,
                      int64_t* reds, double* data);

@@ -19,7 +19,18 @@ class LayoutDump:
         lib = N.hip()
         out = C.c_void_p()
         prec = N.BW_PREC_FP64 if precision == "fp64" else N.BW_PREC_FP32
-        rc = lib.bw_layout_build(C.byref(flat), prec, int(adjoint), int(wi_bytes), C.byref(out))
+        if isinstance(flat, (list, tuple)):  # block-diagonal composite of several meshes
+            flats = list(flat)
+            arr = (C.POINTER(N.FlatHmat) * len(flats))(*[C.pointer(f) for f in flats])
+            rc = lib.bw_layout_build_multi(arr, len(flats), prec, int(adjoint), int(wi_bytes),
+                                           C.byref(out))
+            n_row = sum(f.n_row_pts for f in flats)
+            n_col = sum(f.n_col_pts for f in flats)
+            dof = flats[0].dof
+        else:
+            rc = lib.bw_layout_build(C.byref(flat), prec, int(adjoint), int(wi_bytes),
+                                     C.byref(out))
+            n_row, n_col, dof = flat.n_row_pts, flat.n_col_pts, flat.dof
         if rc != 0:
             raise N.BigWhamError(rc, lib.bw_layout_last_error().decode())
         info = np.zeros(16, dtype=np.int64)
@@ -39,9 +50,9 @@ class LayoutDump:
             self.passes.append(dict(wis=wis[:nwi], gidx=gidx[:ngidx], reds=reds[:nred],
                                     data=data[:ndata], npart=npart, nout=nout))
         lib.bw_layout_free(out)
-        self.dof = flat.dof
-        self.n_in = flat.dof * (flat.n_row_pts if adjoint else flat.n_col_pts)
-        self.n_out = flat.dof * (flat.n_col_pts if adjoint else flat.n_row_pts)
+        self.dof = dof
+        self.n_in = dof * (n_row if adjoint else n_col)
+        self.n_out = dof * (n_col if adjoint else n_row)
 
     def check_bounds(self):
         """Every address k_slab / k_reduce would form lies inside its buffer."""

@@ -17,7 +17,7 @@ ROOT = Path(__file__).resolve().parents[1]
 
 def _declared(header):
     text = (ROOT / "include" / header).read_text()
-    return set(re.findall(r"^\s*(?:int|void|const char\*)\s+(\w+)\s*\(", text, re.M))
+    return set(re.findall(r"^\s*(?:int|void|const char\*|const \w+\s*\*)\s+(\w+)\s*\(", text, re.M))
 
 
 def test_headers_match_exports():

@@ -51,7 +51,7 @@ def test_many_meshes_one_operator(gpu, meshes):
     d = dev.diagonal()
     for m, h in enumerate(meshes):
         sl = slice(off[m], off[m + 1])
-        assert np.array_equal(d[sl], O.diagonal_original(h.flat()))
+        assert np.array_equal(d[sl], O.diagonal(h.flat(), original=True))
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])

@@ -98,6 +98,8 @@ def main():
     ap.add_argument("--gmres", action="store_true", help="also time GMRES to 1e-8")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--wi-bytes", type=int, default=0)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain 1 GiB device copy (calibrates HBM counters)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default unroll depth instead of timing 4/8/16 first")
     args = ap.parse_args()
@@ -108,6 +110,16 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.copy_probe:
+        n = 1 << 27  # 1 GiB of float64 read and 1 GiB written per copy
+        a = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"probe": "copy", "bytes_read": n * 8, "bytes_written": n * 8,
+                          "launches": 5}))
+        return
     from bigwham_dev_amd import HipHmat, HostHmat, set_tuning
     from bigwham_dev_amd import solvers
 

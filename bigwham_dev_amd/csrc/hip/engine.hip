@@ -374,14 +374,23 @@ struct DevOp {
   const int64_t* perm_out = nullptr;  // device, n_out / dof
   int64_t stream_bytes = 0;
   int64_t part_elems[2] = {0, 0};
-  int unroll = 8;             // loads in flight per lane in k_slab (4, 8 or 16)
+  int unroll = 16;            // loads in flight per lane in k_slab (4, 8 or 16)
   // many right-hand sides (FP32 MFMA path), allocated on first use
   float* xb = nullptr;        // xt_size x kRhs
   float* partb[2] = {nullptr, nullptr};
 };
 
-std::atomic<int64_t> g_wi_bytes64{96 * 1024};
-std::atomic<int64_t> g_wi_bytes32{96 * 1024};
+// Work-item bytes for the next create; 0 = sized from the matrix (auto_wi_bytes).
+std::atomic<int64_t> g_wi_bytes64{0};
+std::atomic<int64_t> g_wi_bytes32{0};
+
+// About 32k work items: a few rounds of 8 waves on each of the 1024 SIMDs,
+// capped where a sweep of the 600k-DoF 3DT0 matrix on MI355X stopped gaining
+// (384 KiB for FP64, 192 KiB for FP32) and floored at 8 KiB for small matrices.
+int64_t auto_wi_bytes(int64_t entries, int elem_bytes) {
+  const int64_t cap = elem_bytes == 8 ? (384 << 10) : (192 << 10);
+  return std::min<int64_t>(cap, std::max<int64_t>(8 << 10, entries * elem_bytes / 32768));
+}
 
 template <typename T>
 int upload_pass(const bwl::Layout& L, const bwl::Program& P, DevPass* dp, hipStream_t st,
@@ -529,6 +538,18 @@ int build_op(bw_hip_handle* hd, const bw_flat_hmat* h, bool adjoint, hipStream_t
   opt.vec_elems = Vec<T>::V;
   opt.elem_bytes = sizeof(T);
   opt.wi_bytes = sizeof(T) == 8 ? g_wi_bytes64.load() : g_wi_bytes32.load();
+  if (opt.wi_bytes == 0) {
+    int64_t E = 0;
+    for (int64_t k = 0; k < h->n_fr; ++k) {
+      const int64_t* c = h->fr_pattern + 6 * k;
+      E += h->dof * (c[3] - c[1]) * h->dof * (c[4] - c[2]);
+    }
+    for (int64_t k = 0; k < h->n_lr; ++k) {
+      const int64_t* c = h->lr_pattern + 6 * k;
+      E += c[5] * h->dof * ((c[3] - c[1]) + (c[4] - c[2]));
+    }
+    opt.wi_bytes = auto_wi_bytes(E, sizeof(T));
+  }
   bwl::Layout L;
   bwl::build_layout(view, opt, &L);
   int rc;
@@ -697,8 +718,8 @@ const char* bw_hip_last_error(void) { return g_err.c_str(); }
 
 int bw_hip_set_tuning(int64_t wi64, int64_t wi32) {
   if (wi64 < 0 || wi32 < 0) return fail(BW_EINVAL, "set_tuning: negative size");
-  g_wi_bytes64 = wi64 ? wi64 : 96 * 1024;
-  g_wi_bytes32 = wi32 ? wi32 : 96 * 1024;
+  g_wi_bytes64 = wi64;
+  g_wi_bytes32 = wi32;
   return BW_OK;
 }
 

@@ -44,7 +44,8 @@ def gmres(matvec, b: torch.Tensor, precond=None, x0: torch.Tensor | None = None,
 
     Each iteration synchronises with the host once (the Hessenberg column and
     the new basis norm come back together); the basis vector itself is
-    normalised on the device.  A restart recomputes the residual b - A x with
+    normalised on the device, and on the GPU the next matvec is already queued
+    when the host waits.  A restart recomputes the residual b - A x with
     one matvec, or, with ``implicit_restart``, forms it from the Arnoldi
     relation r = V_{m+1} (beta e_1 - Hbar y) without a matvec (used by the
     inner solves of the mixed-precision refinement, whose outer loop
@@ -59,6 +60,13 @@ def gmres(matvec, b: torch.Tensor, precond=None, x0: torch.Tensor | None = None,
     its = 0
     rel = 1.0
     V = torch.empty((restart + 1, n), dtype=dt, device=dev)
+    # On the GPU the Hessenberg column comes back through pinned memory behind
+    # an event, and the next matvec is queued before the host waits for it, so
+    # the device is not left idle while the host applies the rotations.
+    pipelined = b.is_cuda
+    if pipelined:
+        col_host = torch.empty(restart + 2, dtype=dt, pin_memory=True)
+        ev = torch.cuda.Event()
     r = b.clone() if x0 is None else b - matvec(x)
     while its < maxiter:
         beta = torch.linalg.vector_norm(r).item()
@@ -73,8 +81,12 @@ def gmres(matvec, b: torch.Tensor, precond=None, x0: torch.Tensor | None = None,
         g[0] = beta
         k_done = 0
         breakdown = False
+        pending = None
         for k in range(restart):
-            w = matvec(M(V[k]))
+            if pending is not None:
+                w, pending = pending, None
+            else:
+                w = matvec(M(V[k]))
             its += 1
             # CGS2: two classical Gram-Schmidt passes as matrix-vector products
             Vk = V[: k + 1]
@@ -84,7 +96,19 @@ def gmres(matvec, b: torch.Tensor, precond=None, x0: torch.Tensor | None = None,
             w = torch.addmv(w, Vk.t(), h2, alpha=-1.0)
             hn_t = torch.linalg.vector_norm(w)
             torch.div(w, hn_t, out=V[k + 1])
-            col = torch.cat([h + h2, hn_t.reshape(1)]).tolist()  # the one sync
+            col_t = torch.cat([h + h2, hn_t.reshape(1)])
+            if pipelined:
+                col_host[: k + 2].copy_(col_t, non_blocking=True)
+                ev.record()
+                # look ahead unless the cycle ends here or the last estimate is
+                # already near the tolerance (a look-ahead past convergence is
+                # one wasted matvec)
+                if k + 1 < restart and its < maxiter and rel > 1.5 * tol:
+                    pending = matvec(M(V[k + 1]))
+                ev.synchronize()
+                col = col_host[: k + 2].tolist()
+            else:
+                col = col_t.tolist()
             hn = col[k + 1]
             for i in range(k + 2):
                 Hs[i][k] = Hraw[i][k] = col[i]

@@ -110,7 +110,8 @@ int bw_flat_composite_offsets(const bw_flat_composite* c, int64_t* row_off, int6
 const char* bw_flat_compose_last_error(void);
 void bw_flat_composite_free(bw_flat_composite* c);
 
-/* Tuning knobs for the next create (work-item target bytes; 0 = default). */
+/* Tuning knobs for the next create: work-item target bytes (0 = sized from
+ * the matrix: about 32k work items, 8 KiB to 384 KiB FP64 / 192 KiB FP32). */
 int bw_hip_set_tuning(int64_t wi_bytes_fp64, int64_t wi_bytes_fp32);
 
 /* Loads in flight per lane in the slab kernels of this handle (4, 8 or 16;

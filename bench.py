@@ -43,6 +43,29 @@ CONFIGS = {
 }
 
 
+# HBM bytes per launch of the slab kernels, measured with rocprofv3 PMC counters
+# (FETCH_SIZE and WRITE_SIZE, scaled by a copy of known size) in separate runs;
+# the counters cannot be read in the timed run itself.
+TRAFFIC_FILES = {"c3": "profiles/hbm_traffic_3dt0_600k_dof.md"}
+
+
+def load_traffic(config):
+    """{(precision, pass): HBM bytes per launch} from the PMC summary, if any."""
+    import re
+    path = TRAFFIC_FILES.get(config)
+    if not path or not (ROOT / path).exists():
+        return {}
+    m = re.search(r"```json\n(.*?)```", (ROOT / path).read_text(), re.S)
+    if not m:
+        return {}
+    out = {}
+    for name, v in json.loads(m.group(1))["traffic_bytes"].items():
+        k = re.match(r"k_slab<(\w+), \w+, \d+, (\d)>", name)
+        if k:
+            out[("fp64" if k.group(1) == "double" else "fp32", int(k.group(2)))] = int(v)
+    return out
+
+
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
@@ -176,16 +199,20 @@ def main():
     log(f"fp64 kernels {st64}")
     log(f"fp32 kernels {st32}")
 
+    traffic = load_traffic(args.config)
+
     def roofline(stats, sm, sv):
         # algorithmic bytes per launch of each slab kernel
         b1 = sm * e_b + sv * (n + T)
-        b2 = sm * (e_fr + e_a) + sv * (n + T + n)
+        b2 = sm * (e_fr + e_a) + sv * (n + T) + 8 * n  # y is written in float64
         k1, k2 = stats["p1_slab"], stats["p2_slab"]
         dom, b, ms = ("k_slab pass 2", b2, k2) if k2 >= k1 else ("k_slab pass 1", b1, k1)
+        tr = traffic.get(("fp64" if sm == 8 else "fp32", int(dom[-1])))
         ach = b / (ms * 1e-3) / 1e9
         whole = (sm * E + sv * 4 * n + 2 * 8 * n / host.dof) / (stats["total"] * 1e-3) / 1e9
         return dict(bound="hbm", kernel=dom, achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(ach / HBM_PEAK_GBS, 4), traffic=None,
+                    frac=round(ach / HBM_PEAK_GBS, 4), traffic=tr,
+                    traffic_source=TRAFFIC_FILES.get(args.config) if tr else None,
                     avg_kernel_ms=round(ms, 4), alg_bytes=int(b),
                     whole_matvec_GBs=round(whole, 1),
                     whole_matvec_frac=round(whole / HBM_PEAK_GBS, 4))

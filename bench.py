@@ -38,6 +38,11 @@ CONFIGS = {
                workload="3DR0-H square crack 200x200 rectangles (120000 DoF), leaf 64, eta 3, ACA eps 1e-6"),
     "c3": dict(kernel="3DT0-H", mesh=("penny", 105000), leaf=64, eta=3.0, eps=1e-6,
                workload="3DT0-H penny crack r=1, ~200k Delaunay triangles (~600k DoF), leaf 64, eta 3, ACA eps 1e-6"),
+    "c4": dict(kernel="3DR0-H", mesh=("grid", 224), leaf=64, eta=3.0, eps=1e-6, nrhs=32,
+               n_meshes=16, mesh_elems=(1000, 40000),
+               workload="3DR0-H square crack 224x224 rectangles (150528 DoF) x 32 right-hand "
+                        "sides on FP32 MFMA, plus 16 square cracks of 1k-40k rectangles "
+                        "(log-uniform, seed 7) as one operator; leaf 64, eta 3, ACA eps 1e-6"),
     "c5": dict(kernel="3DR0-H-mode1", mesh=("grid", 500), leaf=64, eta=3.0, eps=1e-6,
                workload="3DR0-H-mode1 500x500 rectangles (250000 DoF), leaf 64, eta 3, ACA eps 1e-6"),
 }
@@ -147,6 +152,10 @@ def main():
     from bigwham_dev_amd import solvers
 
     cfg = CONFIGS[args.config]
+    if args.config == "c4":
+        return bench_batching(args, cfg)
+    if args.config == "c5":
+        return bench_adjoint(args, cfg)
     coor, conn = make_mesh(*cfg["mesh"])
     t0 = time.perf_counter()
     host = HostHmat.from_mesh(cfg["kernel"], coor, conn, [1.0, 0.25])
@@ -266,9 +275,197 @@ def main():
                       "fp32_matvecs": infom.matvecs_fp32, "fp64_matvecs": infom.matvecs_fp64,
                       "rel_residual_fp64": infom.rel_residual},
             "speedup_mixed_vs_fp64": round(info64.seconds / infom.seconds, 3),
+            "matvec_time_ratio_fp64_fp32": round(dt64 / dt32, 4),
             "solution_rel_diff": float(torch.linalg.vector_norm(xm - x64) /
                                        torch.linalg.vector_norm(x64)),
         }
+        if "cpu_baseline" in out:
+            # the host path would run the same FP64 GMRES (scipy on BEMatrix) with
+            # one OpenMP matvec per iteration: bounded by its matvec count times
+            # the measured host matvec time (Krylov work on the host not counted)
+            t_host = info64.matvecs_fp64 / out["cpu_baseline"]["value"]
+            out["gmres"]["host_reference_estimate"] = {
+                "seconds_lower_bound": round(t_host, 2),
+                "basis": f"{info64.matvecs_fp64} FP64 matvecs x host matvec time",
+                "speedup_mixed_vs_host": round(t_host / infom.seconds, 1)}
+    print(json.dumps(out), flush=True)
+
+
+def _build_host(kernel, coor, conn, cfg):
+    from bigwham_dev_amd import HostHmat
+    h = HostHmat.from_mesh(kernel, coor, conn, [1.0, 0.25])
+    h.build(cfg["leaf"], cfg["eta"], cfg["eps"])
+    return h
+
+
+def _timed(fn, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def bench_batching(args, cfg):
+    """Many right-hand sides (FP32 MFMA thin GEMM) and many meshes in one
+    operator.  value = right-hand sides per second of the 32-RHS FP32 call."""
+    import torch
+    from bigwham_dev_amd import HipHmat
+    from bigwham_dev_amd import meshes
+
+    coor, conn = make_mesh(*cfg["mesh"])
+    t0 = time.perf_counter()
+    host = _build_host(cfg["kernel"], coor, conn, cfg)
+    t_build = time.perf_counter() - t0
+    n, E, k = host.size, host.nb_entries(), cfg["nrhs"]
+    log(f"c4 batch matrix: N={n} E={E} build {t_build:.1f}s")
+    dev = HipHmat(host, precisions=("fp64", "fp32"), device=0)
+    rng = np.random.default_rng(1234)
+    X = torch.from_numpy(rng.uniform(-1.0, 1.0, (n, k))).to("cuda:0")
+    steps, warmup = args.steps, args.warmup
+    t_b32 = _timed(lambda: dev.matvec_batched(X, precision="fp32"), steps, warmup)
+    x1 = X[:, 0].contiguous()
+    y1 = torch.empty(n, dtype=torch.float64, device="cuda:0")
+    t_s32 = _timed(lambda: dev.matvec(x1, precision="fp32", out=y1), steps, warmup)
+    t_s64 = _timed(lambda: dev.matvec(x1, precision="fp64", out=y1), steps, warmup)
+    Yb = dev.matvec_batched(X, precision="fp32")
+    ys = torch.stack([dev.matvec(X[:, j].contiguous(), precision="fp32") for j in (0, k - 1)], 1)
+    ys64 = torch.stack([dev.matvec(X[:, j].contiguous()) for j in (0, k - 1)], 1)
+    err = float(torch.linalg.matrix_norm(Yb[:, [0, k - 1]] - ys64) / torch.linalg.matrix_norm(ys64))
+    flops = 2.0 * k * E
+    bytes_b = 4 * E + k * 8 * 2 * n
+    rhs_s = k / t_b32
+    log(f"32-RHS fp32 batched {t_b32 * 1e3:.3f} ms, single fp32 {t_s32 * 1e3:.3f} ms, "
+        f"single fp64 {t_s64 * 1e3:.3f} ms; rel err vs fp64 {err:.2e}")
+
+    # 16 uneven square cracks in one operator
+    mrng = np.random.default_rng(7)
+    lo, hi = cfg["mesh_elems"]
+    counts = np.exp(mrng.uniform(np.log(lo), np.log(hi), cfg["n_meshes"]))
+    sides = [max(4, int(round(np.sqrt(c)))) for c in counts]
+    t0 = time.perf_counter()
+    hosts = [_build_host(cfg["kernel"], *meshes.rect_grid(m), cfg) for m in sides]
+    t_mbuild = time.perf_counter() - t0
+    sizes = [h.size for h in hosts]
+    log(f"16 meshes: sides {sides}, DoF {sum(sizes)}, build {t_mbuild:.1f}s")
+    multi = HipHmat.many(hosts, precisions=("fp64", "fp32"), device=0)
+    singles = [HipHmat(h, precisions=("fp64", "fp32"), device=0) for h in hosts]
+    ntot = int(multi.row_offsets[-1])
+    xm = torch.from_numpy(rng.uniform(-1.0, 1.0, ntot)).to("cuda:0")
+    res = {}
+    for prec in ("fp64", "fp32"):
+        g = multi.graph(xm, precision=prec)
+        t_graph = _timed(g.launch, steps, warmup)
+        ym = torch.empty(ntot, dtype=torch.float64, device="cuda:0")
+        t_multi = _timed(lambda: multi.matvec(xm, precision=prec, out=ym), steps, warmup)
+        parts = [xm[multi.col_offsets[i]:multi.col_offsets[i + 1]] for i in range(len(hosts))]
+        outs = [torch.empty_like(p) for p in parts]
+
+        def sep():
+            for d, p, o in zip(singles, parts, outs):
+                d.matvec(p, precision=prec, out=o)
+        t_sep = _timed(sep, steps, warmup)
+        sep()
+        yg = g.launch().clone()
+        torch.cuda.synchronize()
+        diff = float(torch.linalg.vector_norm(yg - torch.cat(outs)) /
+                     torch.linalg.vector_norm(torch.cat(outs)))
+        res[prec] = {"graph_ms": round(t_graph * 1e3, 4), "one_operator_ms": round(t_multi * 1e3, 4),
+                     "16_separate_ms": round(t_sep * 1e3, 4),
+                     "speedup_graph_vs_separate": round(t_sep / t_graph, 2),
+                     "rel_diff_vs_separate": diff}
+        g.close()
+    e_multi = sum(h.nb_entries() for h in hosts)
+    out = {
+        "metric": "batched H-matvec RHS/s", "value": round(rhs_s, 2), "unit": "RHS/s",
+        "n_gpus": 1, "steps": steps, "warmup": warmup,
+        "ms_per_step": round(t_b32 * 1e3, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": cfg["workload"], "n_dof": n, "nb_entries": E, "nrhs": k},
+        "roofline": {"bound": "mfma", "achieved": round(flops / t_b32 / 1e12, 2),
+                     "peak": 157.3, "unit": "TFLOP/s",
+                     "frac": round(flops / t_b32 / 1e12 / 157.3, 4), "traffic": None,
+                     "hbm_GBs": round(bytes_b / t_b32 / 1e9, 1)},
+        "single_rhs": {"fp32_ms": round(t_s32 * 1e3, 4), "fp64_ms": round(t_s64 * 1e3, 4),
+                       "batched_speedup_vs_32_fp32_calls": round(k * t_s32 / t_b32, 2)},
+        "parity": {"batched_fp32_rel_err_vs_fp64": err},
+        "multi_mesh": {"sides": sides, "dof": sizes, "nb_entries": e_multi, **res},
+        "setup_s": {"host_build": round(t_build, 2), "multi_build": round(t_mbuild, 2)},
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_adjoint(args, cfg):
+    """Gradient of ||H w - t_obs||^2 through the adjoint kernel (autograd).
+    value = gradient evaluations (one H and one H^T matvec) per second."""
+    import torch
+    from bigwham_dev_amd import HipHmat
+    from bigwham_dev_amd.autograd import hmatvec
+
+    coor, conn = make_mesh(*cfg["mesh"])
+    t0 = time.perf_counter()
+    host = _build_host(cfg["kernel"], coor, conn, cfg)
+    t_build = time.perf_counter() - t0
+    n, E = host.size, host.nb_entries()
+    log(f"c5: N={n} E={E} build {t_build:.1f}s")
+    dev = HipHmat(host, precisions=("fp64",), adjoint=True, device=0)
+    xc = torch.from_numpy(host.collocation_points()).to("cuda:0")
+    r2 = (xc[:, 0] ** 2 + xc[:, 1] ** 2)
+    w_true = torch.exp(-r2 / (2 * 0.3 ** 2))  # Gaussian-bump opening
+    t_clean = dev.matvec(w_true)
+    g = torch.Generator(device="cuda:0").manual_seed(11)
+    t_obs = t_clean + 0.01 * t_clean.std() * torch.randn(n, generator=g, device="cuda:0",
+                                                          dtype=torch.float64)
+
+    def loss_fn(w):
+        r = hmatvec(dev, w) - t_obs
+        return torch.dot(r, r)
+
+    w = torch.zeros(n, dtype=torch.float64, device="cuda:0", requires_grad=True)
+
+    def step():
+        w.grad = None
+        loss_fn(w).backward()
+    t_grad = _timed(step, args.steps, args.warmup)
+    # central differences on 20 random directions at a generic point
+    w0 = (w_true + 0.1 * torch.randn(n, generator=g, device="cuda:0", dtype=torch.float64))
+    w0.requires_grad_(True)
+    L0 = loss_fn(w0)
+    L0.backward()
+    grad = w0.grad.detach()
+    errs = []
+    with torch.no_grad():
+        for _ in range(20):
+            d = torch.randn(n, generator=g, device="cuda:0", dtype=torch.float64)
+            eps = 1e-3 * float(torch.linalg.vector_norm(w0) / torch.linalg.vector_norm(d))
+            fd = (loss_fn(w0 + eps * d) - loss_fn(w0 - eps * d)) / (2 * eps)
+            an = torch.dot(grad, d)
+            errs.append(float(abs(fd - an) / abs(an)))
+    # adjoint consistency <H x, y> = <x, H^T y>
+    x = torch.randn(n, generator=g, device="cuda:0", dtype=torch.float64)
+    y = torch.randn(n, generator=g, device="cuda:0", dtype=torch.float64)
+    lhs = float(torch.dot(dev.matvec(x), y))
+    rhs = float(torch.dot(x, dev.rmatvec(y)))
+    bytes_step = 2 * 8 * E
+    out = {
+        "metric": "loss+gradient evaluations/s", "value": round(1.0 / t_grad, 3),
+        "unit": "grad/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(t_grad * 1e3, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": cfg["workload"], "n_dof": n, "nb_entries": E,
+                   "loss": "||H w - t_obs||^2, t_obs = H w_gauss + 1% noise"},
+        "roofline": {"bound": "hbm", "achieved": round(bytes_step / t_grad / 1e9, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(bytes_step / t_grad / 1e9 / HBM_PEAK_GBS, 4),
+                     "traffic": None},
+        "fd_check": {"directions": 20, "max_rel_err": max(errs), "median_rel_err": float(np.median(errs))},
+        "adjoint_identity_rel": abs(lhs - rhs) / abs(lhs),
+        "setup_s": {"host_build": round(t_build, 2)},
+    }
     print(json.dumps(out), flush=True)
 
 

@@ -22,7 +22,7 @@ ARCH = "gfx950"
 
 HOST_SOURCES = sorted((CSRC / "host").glob("hb_*.cpp"))
 HIP_SOURCES = [CSRC / "hip" / "engine.hip", CSRC / "hip" / "layout.cpp",
-               CSRC / "hip" / "layout_capi.cpp", CSRC / "hip" / "composite.cpp", CSRC / "hip" / "batched.hip"]
+               CSRC / "hip" / "layout_capi.cpp", CSRC / "hip" / "composite.cpp"]
 
 
 def _run(cmd: list[str]) -> None:

@@ -280,22 +280,38 @@ __global__ __launch_bounds__(256) void k_slab_mfma(const bwl::WorkItem* __restri
   const int32_t* __restrict__ gi = gidx + wi.gbeg;
   floatx16 acc0 = {0}, acc1 = {0};
   const bool two = R > 32;
-  for (int g = 0; g < ng; g += 2) {
-    const int gg = g + half;
-    const bool has = gg < ng;
-    flt4 a0 = {0.f, 0.f, 0.f, 0.f}, a1 = {0.f, 0.f, 0.f, 0.f};
-    int z = 0;
-    if (has) {
-      z = gi[gg];
-      if (r0 < R) a0 = __builtin_nontemporal_load(base + gg * R + r0);
-      if (two && r0 + 32 < R) a1 = __builtin_nontemporal_load(base + gg * R + 32 + r0);
-    }
+  // kU group pairs per step: their A vectors and input offsets are loaded
+  // first, then the B values, then the MFMAs, so that each wavefront keeps
+  // kU * 16 bytes (or 32 for R > 32) of factors per lane in flight.
+  constexpr int kU = 4;
+  for (int g0 = 0; g0 < ng; g0 += 2 * kU) {
+    flt4 a0[kU], a1[kU];
+    int z[kU];
 #pragma unroll
-    for (int s = 0; s < 4; ++s) {
-      const float b = has ? XB[static_cast<int64_t>(z + s) * kRhs + r0] : 0.0f;
-      acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0[s], b, acc0, 0, 0, 0);
-      if (two) acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1[s], b, acc1, 0, 0, 0);
+    for (int u = 0; u < kU; ++u) {
+      const int gg = g0 + 2 * u + half;
+      const bool has = gg < ng;
+      z[u] = has ? gi[gg] : -1;
+      a0[u] = flt4{0.f, 0.f, 0.f, 0.f};
+      a1[u] = flt4{0.f, 0.f, 0.f, 0.f};
+      if (has) {
+        if (r0 < R) a0[u] = __builtin_nontemporal_load(base + gg * R + r0);
+        if (two && r0 + 32 < R) a1[u] = __builtin_nontemporal_load(base + gg * R + 32 + r0);
+      }
     }
+    float b[kU][4];
+#pragma unroll
+    for (int u = 0; u < kU; ++u)
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+        b[u][s] = z[u] >= 0 ? XB[static_cast<int64_t>(z[u] + s) * kRhs + r0] : 0.0f;
+#pragma unroll
+    for (int u = 0; u < kU; ++u)
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0[u][s], b[u][s], acc0, 0, 0, 0);
+        if (two) acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1[u][s], b[u][s], acc1, 0, 0, 0);
+      }
   }
   const int n = lane & 31;
   auto row_of = [&](int j) { return 8 * (j >> 2) + 4 * half + (j & 3); };

@@ -130,6 +130,12 @@ def hip() -> C.CDLL:
         lib.bw_flat_compose_last_error.restype = C.c_char_p
         lib.bw_flat_composite_free.argtypes = [_vp]
         lib.bw_flat_composite_free.restype = None
+        lib.bw_hip_create_selection.argtypes = [C.POINTER(FlatHmat), _i64, C.c_int64, _i64,
+                                                C.c_int64, C.c_int, C.c_int, C.c_int, C.c_int,
+                                                C.POINTER(_vp)]
+        lib.bw_flat_full_blocks.argtypes = [C.POINTER(FlatHmat), C.c_int, _i64, C.c_int64, _i64,
+                                            C.c_int64, _i64, _d, _i64, _i64]
+        lib.bw_flat_full_blocks_last_error.restype = C.c_char_p
         _hip = lib
     return _hip
 
@@ -165,4 +171,6 @@ HIP_SYMBOLS = ["bw_hip_create", "bw_hip_matvec", "bw_hip_rmatvec", "bw_hip_matve
                "bw_layout_last_error", "bw_layout_build_multi", "bw_hip_create_multi",
                "bw_hip_graph_create", "bw_hip_graph_launch", "bw_hip_graph_destroy",
                "bw_flat_compose", "bw_flat_composite_view", "bw_flat_composite_offsets",
-               "bw_flat_compose_last_error", "bw_flat_composite_free"]
+               "bw_flat_compose_last_error", "bw_flat_composite_free",
+               "bw_hip_create_selection", "bw_flat_full_blocks",
+               "bw_flat_full_blocks_last_error"]

@@ -4,8 +4,9 @@ interfaces/python/bigwham4py/bigwham4py.py:49-485) on the MI355X engine.
 Same constructor arguments, same LinearOperator semantics (shape (N, N),
 float64, matvec in the original DoF ordering), same helper methods for the
 matvec path: getCompression, getPermutation, getCollocationPoints,
-getSpatialDimension, H_diag, H_jacobi_prec, get_diagonal, getMatvecTime,
-get_max_error_ACA, _getPattern.  Construction stays on the host (libbwhost);
+getSpatialDimension, H_diag, H_jacobi_prec, H_ILU_prec, _getFullBlocks,
+get_diagonal, getMatvecTime, get_max_error_ACA, _getPattern, and
+sub-matrix selection by indexing (M[np.ix_(rows, cols)]).  Construction stays on the host (libbwhost);
 the matvec runs on the GPU (libbwhip).  `useCuda` is accepted for signature
 compatibility; there is no CPU matvec fallback in this package.
 
@@ -18,10 +19,10 @@ from __future__ import annotations
 import time
 
 import numpy as np
-from scipy.sparse import diags
-from scipy.sparse.linalg import LinearOperator
+from scipy.sparse import csc_matrix, diags
+from scipy.sparse.linalg import LinearOperator, spilu
 
-from .hmat import HipHmat, HostHmat
+from .hmat import HipHmat, HostHmat, full_blocks
 
 kernels_id = ["2DS0-H", "3DT0-H", "3DR0-H", "3DR0-H-mode1"]
 
@@ -121,10 +122,59 @@ class BEMatrix(LinearOperator):
                 np.stack([lp[1], lp[2], lp[3], lp[4], np.ones_like(lp[1]), lp[5]], 1)]
         return np.concatenate(rows, 0)
 
+    def _getFullBlocks(self, original_order: bool = True, keep_ratio: float = 1.0) -> csc_matrix:
+        """Entries of the full-rank blocks as a sparse matrix
+        (bigwham4py.py:252-276): all of them, or the largest keep_ratio
+        fraction by magnitude."""
+        if not (0 < keep_ratio <= 1):
+            raise ValueError("ratio must be a float between 0 and 1.")
+        sel = getattr(self, "_selection", None)
+        if sel is None:
+            val, row, col = full_blocks(self.H_, original=original_order)
+        else:
+            val, row, col = full_blocks(self.H_, original=True, rows=sel[0], cols=sel[1])
+        if keep_ratio < 1.0:
+            k = int(np.ceil(keep_ratio * val.size))
+            if k == 0:
+                return csc_matrix(self.shape_)
+            thr = np.partition(np.abs(val), -k)[-k]
+            m = np.abs(val) >= thr
+            val, row, col = val[m], row[m], col[m]
+        return csc_matrix((val, (row, col)), shape=self.shape_)
+
+    def H_ILU_prec(self, fill_factor=1, drop_tol=1e-3, keep_ratio=1.0) -> LinearOperator:
+        """ILU (scipy spilu) of the full-rank blocks (bigwham4py.py:345-361)."""
+        self._build()
+        ilu = spilu(self._getFullBlocks(keep_ratio=keep_ratio), fill_factor=fill_factor,
+                    drop_tol=drop_tol)
+        return LinearOperator(self.shape_, ilu.solve)
+
+    def __getitem__(self, indices):
+        """Sub-matrix selection (bigwham4py.py:462-483): ``M[np.ix_(rows, cols)]``
+        or ``M[rows, cols]`` with point indices in the original numbering.  The
+        result is a BEMatrix whose matvec runs on the GPU over the blocks that
+        meet the selection only."""
+        if not isinstance(indices, tuple) or len(indices) != 2:
+            raise IndexError("Subsetting requires indices of the form (np.ix_(row_indices, col_indices))")
+        rows = np.asarray(indices[0]).ravel().astype(np.int64)
+        cols = np.asarray(indices[1]).ravel().astype(np.int64)
+        new = object.__new__(BEMatrix)
+        new.__dict__ = self.__dict__.copy()
+        new.dev_ = HipHmat.selection(self.H_, rows, cols, original=True,
+                                     precisions=self._precisions, adjoint=self._adjoint,
+                                     device=self._device)
+        new._selection = (rows, cols)
+        new.shape_ = (new.dev_.n_rows, new.dev_.n_cols)
+        new.matvec_size_ = new.shape_[0]
+        LinearOperator.__init__(new, float, new.shape_)
+        return new
+
     def get_max_error_ACA(self) -> float:
         return self.H_.max_aca_error()
 
     def get_diagonal(self) -> np.ndarray:
+        if getattr(self, "_selection", None) is not None:
+            raise NotImplementedError("the diagonal of a selection is not defined")
         return self.dev_.diagonal(original=True)
 
     def H_diag(self) -> np.ndarray:

@@ -22,7 +22,8 @@ ARCH = "gfx950"
 
 HOST_SOURCES = sorted((CSRC / "host").glob("hb_*.cpp"))
 HIP_SOURCES = [CSRC / "hip" / "engine.hip", CSRC / "hip" / "layout.cpp",
-               CSRC / "hip" / "layout_capi.cpp", CSRC / "hip" / "composite.cpp"]
+               CSRC / "hip" / "layout_capi.cpp", CSRC / "hip" / "composite.cpp",
+               CSRC / "hip" / "fullblocks.cpp"]
 
 
 def _run(cmd: list[str]) -> None:

@@ -229,6 +229,32 @@ __global__ __launch_bounds__(64) void k_reduce(const bwl::Reduce* __restrict__ r
   out[out_index<OutT>(static_cast<int64_t>(rd.out_row) + t, perm, dof)] = static_cast<OutT>(s);
 }
 
+// Sub-matrix selection: the input is scattered into the (zeroed) x part of xt
+// at the permuted points of the selected columns, and the output gathered
+// from a full permuted y at the selected rows.
+template <typename T>
+__global__ __launch_bounds__(256) void k_scatter_sel(const double* __restrict__ x,
+                                                     T* __restrict__ xt,
+                                                     const int64_t* __restrict__ src,
+                                                     const int64_t* __restrict__ dst, int64_t n,
+                                                     int dof) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n * dof) return;
+  const int64_t p = i / dof;
+  const int64_t c = i - p * dof;
+  xt[dof * dst[p] + c] = static_cast<T>(x[dof * src[p] + c]);
+}
+
+__global__ __launch_bounds__(256) void k_gather_sel(const double* __restrict__ yfull,
+                                                    double* __restrict__ y,
+                                                    const int64_t* __restrict__ dst, int64_t n,
+                                                    int dof) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n * dof) return;
+  const int64_t p = i / dof;
+  y[i] = yfull[dof * dst[p] + (i - p * dof)];
+}
+
 // ---------------------------------------------------------------------------
 // Many right-hand sides on FP32 MFMA.
 //
@@ -517,6 +543,23 @@ int upload_pass(const bwl::Layout& L, const bwl::Program& P, DevPass* dp, hipStr
 
 }  // namespace
 
+// One direction of a selection operator (0: H, 1: H^T).
+struct SelDir {
+  int64_t* in_src = nullptr;  // point of each scattered input in the caller's vector
+  int64_t* in_dst = nullptr;  // its permuted point (repeated indices: the last one wins)
+  int64_t n_pairs = 0;
+  int64_t n_in = 0;           // points in the caller's input vector
+  int64_t* out_dst = nullptr; // permuted point of each output point
+  int64_t n_out = 0;
+};
+
+struct SelInfo {
+  bool on = false;
+  int ordering = BW_ORDER_ORIGINAL;
+  SelDir dir[2];
+  double* y_full = nullptr;  // full permuted output of the selected blocks
+};
+
 struct bw_hip_handle {
   int device = 0;
   int dof = 1;
@@ -530,6 +573,7 @@ struct bw_hip_handle {
   std::vector<double> diag_perm, diag_orig;
   std::vector<void*> allocs;
   int64_t device_bytes = 0;
+  SelInfo sel;
   // Kernel timing: two alternating sets of events between the kernels of a
   // matvec; a set is read (and accumulated) only when reused two calls later,
   // so timing never stalls the launch pipeline.
@@ -630,12 +674,20 @@ void launch_slab(int unroll, int blocks, hipStream_t st, const bwl::WorkItem* wi
 
 template <typename T>
 int launch_matvec(bw_hip_handle* hd, DevOp& op, const double* xd, double* yd, int ordering,
-                  hipStream_t st) {
-  const bool orig = ordering == BW_ORDER_ORIGINAL;
+                  hipStream_t st, const SelDir* sd = nullptr) {
+  const bool orig = !sd && ordering == BW_ORDER_ORIGINAL;
   const int dof = hd->dof;
   T* xt = static_cast<T*>(op.xt);
+  double* const y_user = yd;
+  if (sd) yd = hd->sel.y_full;  // the passes write the full permuted y
   if (hd->timing) hipEventRecord(hd->ev[0], st);
-  {
+  if (sd) {
+    if (sd->n_pairs > 0) {
+      const int blocks = static_cast<int>((sd->n_pairs * dof + 255) / 256);
+      hipLaunchKernelGGL(k_scatter_sel<T>, dim3(blocks), dim3(256), 0, st, xd, xt, sd->in_src,
+                         sd->in_dst, sd->n_pairs, dof);
+    }
+  } else {
     const int64_t n = op.n_in;
     const int blocks = static_cast<int>((n + 255) / 256);
     hipLaunchKernelGGL(k_gather<T>, dim3(blocks), dim3(256), 0, st, xd, xt, n,
@@ -664,6 +716,11 @@ int launch_matvec(bw_hip_handle* hd, DevOp& op, const double* xd, double* yd, in
     hipLaunchKernelGGL((k_reduce<T, double>), dim3(op.p2.nbig), dim3(64), 0, st, op.p2.big_reds,
                        op.p2.nbig, static_cast<const T*>(op.p2.part), yd,
                        orig ? op.perm_out : nullptr, dof);
+  if (sd && sd->n_out > 0) {
+    const int blocks = static_cast<int>((sd->n_out * dof + 255) / 256);
+    hipLaunchKernelGGL(k_gather_sel, dim3(blocks), dim3(256), 0, st, yd, y_user, sd->out_dst,
+                       sd->n_out, dof);
+  }
   if (hd->timing) hipEventRecord(hd->ev[5], st);
   HIPCHK(hipGetLastError());
   return BW_OK;
@@ -695,12 +752,20 @@ int do_matvec(bw_hip_handle* hd, bool adjoint, const double* x, double* y, int o
   if (!op.present)
     return fail(BW_ENOTSUP, adjoint ? "rmatvec: adjoint / precision not built for this handle"
                                     : "matvec: precision not built for this handle");
+  const SelDir* sd = nullptr;
+  if (hd->sel.on) {
+    if (ordering != hd->sel.ordering)
+      return fail(BW_ENOTSUP, "matvec: this selection was created for the other ordering");
+    sd = &hd->sel.dir[adjoint ? 1 : 0];
+  }
+  const int64_t n_in = sd ? hd->dof * sd->n_in : op.n_in;
+  const int64_t n_out = sd ? hd->dof * sd->n_out : op.n_out;
   hipStream_t st = static_cast<hipStream_t>(stream);
   HIPCHK(hipSetDevice(hd->device));
   const double* xd = x;
   double* yd = y;
   if (!on_device) {
-    HIPCHK(hipMemcpyAsync(hd->x_stage, x, op.n_in * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(hd->x_stage, x, n_in * sizeof(double), hipMemcpyHostToDevice, st));
     xd = hd->x_stage;
     yd = hd->y_stage;
   }
@@ -712,15 +777,15 @@ int do_matvec(bw_hip_handle* hd, bool adjoint, const double* x, double* y, int o
     }
     hd->ev = hd->evs[set];
   }
-  int rc = precision == BW_PREC_FP64 ? launch_matvec<double>(hd, op, xd, yd, ordering, st)
-                                     : launch_matvec<float>(hd, op, xd, yd, ordering, st);
+  int rc = precision == BW_PREC_FP64 ? launch_matvec<double>(hd, op, xd, yd, ordering, st, sd)
+                                     : launch_matvec<float>(hd, op, xd, yd, ordering, st, sd);
   if (rc) return rc;
   if (hd->timing) {
     hd->pending[hd->cur] = true;
     hd->cur ^= 1;
   }
   if (!on_device) {
-    HIPCHK(hipMemcpyAsync(y, hd->y_stage, op.n_out * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(y, hd->y_stage, n_out * sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
   }
   return BW_OK;
@@ -872,6 +937,139 @@ int bw_hip_create_multi(const bw_flat_hmat* const* hs, int n_mats, int precision
   return rc;
 }
 
+int bw_hip_create_selection(const bw_flat_hmat* h, const int64_t* rows, int64_t n_rows,
+                            const int64_t* cols, int64_t n_cols, int ordering,
+                            int precision_mask, int flags, int device, bw_hip_handle** out) {
+  if (!out) return fail(BW_EINVAL, "create_selection: null output pointer");
+  *out = nullptr;
+  std::string msg;
+  int rc = bwl::validate_flat(h, &msg);
+  if (rc) return fail(rc, "create_selection: " + msg);
+  if (!rows || !cols || n_rows <= 0 || n_cols <= 0)
+    return fail(BW_EINVAL, "create_selection: empty row or column selection");
+  if (ordering != BW_ORDER_PERMUTED && ordering != BW_ORDER_ORIGINAL)
+    return fail(BW_EINVAL, "create_selection: ordering must be 0 (permuted) or 1 (original)");
+  for (int64_t i = 0; i < n_rows; ++i)
+    if (rows[i] < 0 || rows[i] >= h->n_row_pts)
+      return fail(BW_EINVAL, "create_selection: row index out of bounds");
+  for (int64_t i = 0; i < n_cols; ++i)
+    if (cols[i] < 0 || cols[i] >= h->n_col_pts)
+      return fail(BW_EINVAL, "create_selection: column index out of bounds");
+  // selected points in the permuted numbering (HmatSelection::blockSelection,
+  // hmat_selection.cc:44-77): the inverse permutation for original indices
+  std::vector<int64_t> inv0(h->n_row_pts), inv1(h->n_col_pts);
+  for (int64_t i = 0; i < h->n_row_pts; ++i) inv0[h->perm0[i]] = i;
+  for (int64_t i = 0; i < h->n_col_pts; ++i) inv1[h->perm1[i]] = i;
+  const bool orig = ordering == BW_ORDER_ORIGINAL;
+  std::vector<int64_t> prow(n_rows), pcol(n_cols);
+  for (int64_t i = 0; i < n_rows; ++i) prow[i] = orig ? inv0[rows[i]] : rows[i];
+  for (int64_t i = 0; i < n_cols; ++i) pcol[i] = orig ? inv1[cols[i]] : cols[i];
+  // prefix counts of marked points: a block is kept if its row range holds a
+  // selected row and its column range a selected column (hmat_selection.cc:82-128)
+  std::vector<int64_t> rcnt(h->n_row_pts + 1, 0), ccnt(h->n_col_pts + 1, 0);
+  {
+    std::vector<char> mr(h->n_row_pts, 0), mc(h->n_col_pts, 0);
+    for (int64_t p : prow) mr[p] = 1;
+    for (int64_t p : pcol) mc[p] = 1;
+    for (int64_t i = 0; i < h->n_row_pts; ++i) rcnt[i + 1] = rcnt[i] + mr[i];
+    for (int64_t i = 0; i < h->n_col_pts; ++i) ccnt[i + 1] = ccnt[i] + mc[i];
+  }
+  auto keep = [&](const int64_t* c) {
+    return rcnt[c[3]] > rcnt[c[1]] && ccnt[c[4]] > ccnt[c[2]];
+  };
+  std::vector<int64_t> fr_pat, lr_pat;
+  std::vector<const double*> fr_ptr, lr_a, lr_b;
+  long double area = 0.0L;
+  for (int64_t k = 0; k < h->n_fr; ++k) {
+    const int64_t* c = h->fr_pattern + 6 * k;
+    if (!keep(c)) continue;
+    fr_pat.insert(fr_pat.end(), c, c + 6);
+    fr_ptr.push_back(h->fr_blocks[k]);
+    area += static_cast<long double>(c[3] - c[1]) * (c[4] - c[2]);
+  }
+  for (int64_t k = 0; k < h->n_lr; ++k) {
+    const int64_t* c = h->lr_pattern + 6 * k;
+    if (!keep(c)) continue;
+    lr_pat.insert(lr_pat.end(), c, c + 6);
+    lr_a.push_back(h->lr_A[k]);
+    lr_b.push_back(h->lr_B[k]);
+    area += static_cast<long double>(c[3] - c[1]) * (c[4] - c[2]);
+  }
+  bw_flat_hmat sub = *h;
+  sub.n_fr = static_cast<int64_t>(fr_ptr.size());
+  sub.fr_pattern = sub.n_fr ? fr_pat.data() : nullptr;
+  sub.fr_blocks = sub.n_fr ? fr_ptr.data() : nullptr;
+  sub.n_lr = static_cast<int64_t>(lr_a.size());
+  sub.lr_pattern = sub.n_lr ? lr_pat.data() : nullptr;
+  sub.lr_A = sub.n_lr ? lr_a.data() : nullptr;
+  sub.lr_B = sub.n_lr ? lr_b.data() : nullptr;
+  bw_hip_handle* hd = nullptr;
+  rc = create_impl(&sub, area, precision_mask, flags, device, &hd);
+  if (rc) return rc;
+  // scatter pairs with repeated targets reduced to the last occurrence, as
+  // HmatSelection::matvec overwrites x_full (hmat_selection.cc:197-205)
+  auto pairs = [](const std::vector<int64_t>& dst, std::vector<int64_t>* s, std::vector<int64_t>* d) {
+    std::vector<int64_t> order(dst.size());
+    for (size_t i = 0; i < dst.size(); ++i) order[i] = static_cast<int64_t>(i);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int64_t a, int64_t b) { return dst[a] < dst[b]; });
+    for (size_t k = 0; k < order.size(); ++k)
+      if (k + 1 == order.size() || dst[order[k + 1]] != dst[order[k]]) {
+        s->push_back(order[k]);
+        d->push_back(dst[order[k]]);
+      }
+  };
+  auto up = [&](const std::vector<int64_t>& v, int64_t** dp) -> int {
+    if (hipMalloc(reinterpret_cast<void**>(dp), std::max<size_t>(v.size(), 1) * sizeof(int64_t)) !=
+        hipSuccess)
+      return fail(BW_ENOMEM, "create_selection: hipMalloc");
+    hd->allocs.push_back(*dp);
+    if (!v.empty() &&
+        hipMemcpy(*dp, v.data(), v.size() * sizeof(int64_t), hipMemcpyHostToDevice) != hipSuccess)
+      return fail(BW_EHIP, "create_selection: upload");
+    return BW_OK;
+  };
+  const std::vector<int64_t>* in_of[2] = {&pcol, &prow};
+  const std::vector<int64_t>* out_of[2] = {&prow, &pcol};
+  for (int dir = 0; dir < 2; ++dir) {
+    SelDir& sd = hd->sel.dir[dir];
+    std::vector<int64_t> s_src, s_dst;
+    pairs(*in_of[dir], &s_src, &s_dst);
+    if ((rc = up(s_src, &sd.in_src)) || (rc = up(s_dst, &sd.in_dst)) ||
+        (rc = up(*out_of[dir], &sd.out_dst))) {
+      bw_hip_destroy(hd);
+      return rc;
+    }
+    sd.n_pairs = static_cast<int64_t>(s_src.size());
+    sd.n_in = static_cast<int64_t>(in_of[dir]->size());
+    sd.n_out = static_cast<int64_t>(out_of[dir]->size());
+  }
+  const int64_t nfull = h->dof * std::max(h->n_row_pts, h->n_col_pts);
+  if (hipMalloc(reinterpret_cast<void**>(&hd->sel.y_full), nfull * sizeof(double)) != hipSuccess) {
+    bw_hip_destroy(hd);
+    return fail(BW_ENOMEM, "create_selection: hipMalloc");
+  }
+  hd->allocs.push_back(hd->sel.y_full);
+  // host-pointer staging must hold the selection vectors, which repeated
+  // indices can make longer than the full ones
+  const int64_t nsel = h->dof * std::max(n_rows, n_cols);
+  if (nsel > nfull) {
+    for (double** stage : {&hd->x_stage, &hd->y_stage}) {
+      if (hipMalloc(reinterpret_cast<void**>(stage), nsel * sizeof(double)) != hipSuccess) {
+        bw_hip_destroy(hd);
+        return fail(BW_ENOMEM, "create_selection: hipMalloc");
+      }
+      hd->allocs.push_back(*stage);
+    }
+  }
+  hd->sel.on = true;
+  hd->sel.ordering = ordering;
+  hd->n_row_pts = n_rows;
+  hd->n_col_pts = n_cols;
+  *out = hd;
+  return BW_OK;
+}
+
 int bw_hip_graph_create(bw_hip_handle* hd, const double* x, double* y, int ordering,
                         int precision, int adjoint, bw_hip_graph** out) {
   if (!hd || !x || !y || !out) return fail(BW_EINVAL, "graph_create: null argument");
@@ -891,8 +1089,13 @@ int bw_hip_graph_create(bw_hip_handle* hd, const double* x, double* y, int order
   int rc = BW_OK;
   hipGraph_t graph = nullptr;
   if (e == hipSuccess) {
-    rc = precision == BW_PREC_FP64 ? launch_matvec<double>(hd, op, x, y, ordering, cs)
-                                   : launch_matvec<float>(hd, op, x, y, ordering, cs);
+    const SelDir* sd = hd->sel.on ? &hd->sel.dir[adjoint ? 1 : 0] : nullptr;
+    if (sd && ordering != hd->sel.ordering) {
+      rc = fail(BW_ENOTSUP, "graph_create: this selection was created for the other ordering");
+    } else {
+      rc = precision == BW_PREC_FP64 ? launch_matvec<double>(hd, op, x, y, ordering, cs, sd)
+                                     : launch_matvec<float>(hd, op, x, y, ordering, cs, sd);
+    }
     hipError_t e2 = hipStreamEndCapture(cs, &graph);
     if (!rc && e2 != hipSuccess) rc = fail(BW_EHIP, std::string("graph_create: ") + hipGetErrorString(e2));
   } else {
@@ -946,6 +1149,7 @@ int bw_hip_rmatvec(bw_hip_handle* hd, const double* x, double* y, int ordering, 
 int bw_hip_matvec_batched(bw_hip_handle* hd, int nrhs, const double* X, int64_t ldx, double* Y,
                           int64_t ldy, int ordering, int precision, void* stream) {
   if (!hd || !X || !Y || nrhs <= 0) return fail(BW_EINVAL, "matvec_batched: bad argument");
+  if (hd->sel.on) return fail(BW_ENOTSUP, "matvec_batched: not available for a selection");
   if (ordering != BW_ORDER_PERMUTED && ordering != BW_ORDER_ORIGINAL)
     return fail(BW_EINVAL, "matvec_batched: bad ordering");
   if (precision != BW_PREC_FP64 && precision != BW_PREC_FP32)
@@ -1014,6 +1218,7 @@ int bw_hip_matvec_batched(bw_hip_handle* hd, int nrhs, const double* X, int64_t 
 
 int bw_hip_diagonal(bw_hip_handle* hd, double* dg, int ordering) {
   if (!hd || !dg) return fail(BW_EINVAL, "diagonal: null argument");
+  if (hd->sel.on) return fail(BW_ENOTSUP, "diagonal: not defined for a selection");
   if (ordering != BW_ORDER_PERMUTED && ordering != BW_ORDER_ORIGINAL)
     return fail(BW_EINVAL, "diagonal: bad ordering");
   const auto& v = ordering == BW_ORDER_ORIGINAL ? hd->diag_orig : hd->diag_perm;

@@ -206,6 +206,26 @@ class HipHmat:
         self.row_offsets, self.col_offsets = ro, co
         return self
 
+    @classmethod
+    def selection(cls, host, rows, cols, original: bool = True, precisions=("fp64",),
+                  adjoint: bool = False, device: int = -1):
+        """The sub-matrix H[rows, cols] of point indices (original numbering,
+        or permuted with original=False) as its own device operator
+        (bw_hip_create_selection); its matvec takes the same `original`."""
+        self = cls.__new__(cls)
+        mask, flags = cls._mask(precisions, adjoint)
+        r = np.ascontiguousarray(np.asarray(rows).ravel(), dtype=np.int64)
+        c = np.ascontiguousarray(np.asarray(cols).ravel(), dtype=np.int64)
+        out = C.c_void_p()
+        N.check_hip(N.hip().bw_hip_create_selection(
+            C.byref(host.flat()), N.iptr(r), r.size, N.iptr(c), c.size,
+            N.BW_ORDER_ORIGINAL if original else N.BW_ORDER_PERMUTED, mask, flags,
+            int(device), C.byref(out)))
+        self._host = host
+        self._finish(out, precisions, adjoint, device)
+        self.selection_original = bool(original)
+        return self
+
     @staticmethod
     def _mask(precisions, adjoint):
         mask = 0
@@ -356,6 +376,36 @@ class MatvecGraph:
             self.close()
         except Exception:
             pass
+
+
+def full_blocks(hmat, original: bool = True, rows=None, cols=None):
+    """Entries of the full-rank blocks as (values, row, col) DoF coordinate
+    arrays (bw_flat_full_blocks): the whole matrix, or the entries inside a
+    point selection indexed in that selection.  `hmat` is anything with
+    .flat() (HostHmat, store.FlatFile)."""
+    f = hmat.flat()
+    order = N.BW_ORDER_ORIGINAL if original else N.BW_ORDER_PERMUTED
+    if (rows is None) != (cols is None):
+        raise ValueError("give both rows and cols, or neither")
+    if rows is not None:
+        r = np.ascontiguousarray(np.asarray(rows).ravel(), dtype=np.int64)
+        c = np.ascontiguousarray(np.asarray(cols).ravel(), dtype=np.int64)
+        sel = (N.iptr(r), r.size, N.iptr(c), c.size)
+    else:
+        sel = (None, 0, None, 0)
+    lib = N.hip()
+    n = C.c_int64()
+
+    def check(rc):
+        if rc != N.BW_OK:
+            raise N.BigWhamError(rc, lib.bw_flat_full_blocks_last_error().decode())
+    check(lib.bw_flat_full_blocks(C.byref(f), order, *sel, C.byref(n), None, None, None))
+    vals = np.empty(n.value)
+    ri = np.empty(n.value, dtype=np.int64)
+    ci = np.empty(n.value, dtype=np.int64)
+    check(lib.bw_flat_full_blocks(C.byref(f), order, *sel, C.byref(n), N.dptr(vals),
+                                  N.iptr(ri), N.iptr(ci)))
+    return vals, ri, ci
 
 
 def set_tuning(wi_bytes_fp64: int = 0, wi_bytes_fp32: int = 0) -> None:

@@ -62,6 +62,20 @@ int bw_hip_create_multi(const bw_flat_hmat* const* hs, int n_mats, int precision
                         int flags, int device, bw_hip_handle** out, int64_t* row_off,
                         int64_t* col_off);
 
+/* Sub-matrix selection (replaces HmatSelection, hmat_selection.h:12-84 and
+ * hmat_selection.cc:141-257, reached through BigWhamIO::hmatSelection,
+ * bigwham_io.cc:1365-1375): the operator H[rows, cols] on DoF vectors of
+ * dof * n_cols entries in, dof * n_rows out.  rows / cols are point indices,
+ * in the original numbering for ordering = BW_ORDER_ORIGINAL
+ * (HmatSelection::matvecOriginal) or the permuted one for BW_ORDER_PERMUTED
+ * (HmatSelection::matvec); the handle serves that ordering only.  Only the
+ * blocks that meet both selections are laid out and streamed.  A repeated
+ * column index takes its last input value, as in the reference.  Diagonal
+ * and batched calls return BW_ENOTSUP. */
+int bw_hip_create_selection(const bw_flat_hmat* h, const int64_t* rows, int64_t n_rows,
+                            const int64_t* cols, int64_t n_cols, int ordering,
+                            int precision_mask, int flags, int device, bw_hip_handle** out);
+
 /* y = H x.  precision: BW_PREC_FP64 (FP64 factors, FP64 accumulation) or
  * BW_PREC_FP32 (FP32 factors, FP32 accumulation; x, y stay double).
  * on_device: 1 if x and y are device pointers, 0 for host pointers.
@@ -109,6 +123,20 @@ const bw_flat_hmat* bw_flat_composite_view(const bw_flat_composite* c);
 int bw_flat_composite_offsets(const bw_flat_composite* c, int64_t* row_off, int64_t* col_off);
 const char* bw_flat_compose_last_error(void);
 void bw_flat_composite_free(bw_flat_composite* c);
+
+/* Full-rank block entries as a coordinate list (values[k] = H(row_idx[k],
+ * col_idx[k]), DoF indices), host only.  rows == cols == NULL: every entry,
+ * blocks in pattern order, each column-major, indices in the original
+ * (Hmat::fullBlocksOriginal, hmat.cc:77-140) or permuted
+ * (Hmat::fullBlocksPerm, hmat.cc:143-182) numbering.  With a selection of
+ * points (original or permuted numbering per `ordering`): the entries whose
+ * row and column are both selected, indexed in the selection
+ * (HmatSelection::fullBlocksOriginal, hmat_selection.cc:262-371).  Call once
+ * with values == NULL to get *count, then with arrays of that length. */
+int bw_flat_full_blocks(const bw_flat_hmat* h, int ordering, const int64_t* rows,
+                        int64_t n_rows, const int64_t* cols, int64_t n_cols, int64_t* count,
+                        double* values, int64_t* row_idx, int64_t* col_idx);
+const char* bw_flat_full_blocks_last_error(void);
 
 /* Tuning knobs for the next create: work-item target bytes (0 = sized from
  * the matrix: about 32k work items, 8 KiB to 384 KiB FP64 / 192 KiB FP32). */

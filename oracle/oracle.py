@@ -94,6 +94,36 @@ def rmatvec_original(flat, x: np.ndarray) -> np.ndarray:
     return out
 
 
+def _inverse(perm):
+    inv = np.empty_like(perm)
+    inv[perm] = np.arange(perm.size)
+    return inv
+
+
+def selection_matvec(flat, rows, cols, x, original: bool = True, adjoint: bool = False):
+    """HmatSelection::matvecOriginal / matvec (hmat_selection.cc:188-257):
+    scatter x into a zero permuted vector at the selected columns (a repeated
+    column keeps its last value), multiply by the whole H, gather the
+    selected rows.  Point indices are original (through the inverse
+    permutations) or permuted.  adjoint: the same for H^T, rows and columns
+    swapping roles."""
+    d = flat.dof
+    rows = np.asarray(rows, dtype=np.int64)
+    cols = np.asarray(cols, dtype=np.int64)
+    p0 = np.ctypeslib.as_array(flat.perm0, shape=(flat.n_row_pts,))
+    p1 = np.ctypeslib.as_array(flat.perm1, shape=(flat.n_col_pts,))
+    pr = _inverse(p0)[rows] if original else rows
+    pc = _inverse(p1)[cols] if original else cols
+    src, dst = (pr, pc) if adjoint else (pc, pr)
+    x = np.asarray(x, dtype=np.float64).reshape(-1, d)
+    n_in = d * (flat.n_row_pts if adjoint else flat.n_col_pts)
+    xf = np.zeros((n_in // d, d))
+    for i, p in enumerate(src):  # in order: the last occurrence wins
+        xf[p] = x[i]
+    yf = (rmatvec if adjoint else matvec)(flat, xf.ravel()).reshape(-1, d)
+    return yf[dst].ravel()
+
+
 def diagonal(flat, original: bool = False) -> np.ndarray:
     """Hmat<T>::diagonal / diagonalOriginal (hmat.cc:237-285)."""
     d = flat.dof

@@ -47,8 +47,14 @@ def test_selection_matvec(gpu, host, original):
     reft = O.selection_matvec(f, rows, cols, xr, original=original, adjoint=True)
     yt = dev.rmatvec(torch.from_numpy(xr).to(gpu), original=original).cpu().numpy()
     assert rel(yt, reft) <= 1e-12
-    # only the blocks meeting the selection are stored
-    assert dev.nb_entries < host.nb_entries()
+    # only the blocks meeting the selection are stored: a cluster of points
+    # touches a fraction of the blocks (random points touch them all)
+    near = host.permutation()[:20] if original else np.arange(20)
+    small = HipHmat.selection(host, near, near, original=original)
+    assert small.nb_entries < host.nb_entries() // 4
+    xs = rng.uniform(-1, 1, d * near.size)
+    ys = small.matvec(torch.from_numpy(xs).to(gpu), original=original).cpu().numpy()
+    assert rel(ys, O.selection_matvec(f, near, near, xs, original=original)) <= 1e-12
     with pytest.raises(BigWhamError, match="other ordering"):
         dev.matvec(xd, original=not original)
     with pytest.raises(BigWhamError, match="selection"):

@@ -284,7 +284,7 @@ __global__ __launch_bounds__(256) void k_gather_b(const double* __restrict__ X, 
 }
 
 template <int PASS>
-__global__ __launch_bounds__(256) void k_slab_mfma(const bwl::WorkItem* __restrict__ wis, int nwi,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_slab_mfma(const bwl::WorkItem* __restrict__ wis, int nwi,
                                                    const int32_t* __restrict__ gidx,
                                                    const float* __restrict__ data,
                                                    const float* __restrict__ XB,
@@ -306,21 +306,30 @@ __global__ __launch_bounds__(256) void k_slab_mfma(const bwl::WorkItem* __restri
   const int32_t* __restrict__ gi = gidx + wi.gbeg;
   floatx16 acc0 = {0}, acc1 = {0};
   const bool two = R > 32;
-  // kU group pairs per step: their A vectors and input offsets are loaded
-  // first, then the B values, then the MFMAs, so that each wavefront keeps
-  // kU * 16 bytes (or 32 for R > 32) of factors per lane in flight.
+  // kU group pairs per step: their A vectors and B values are loaded first,
+  // then the MFMAs run.  The input offsets of the next step are loaded one
+  // step ahead, so the B loads never wait on an offset load.
   constexpr int kU = 4;
+  int zc[kU];
+#pragma unroll
+  for (int u = 0; u < kU; ++u) {
+    const int gg = 2 * u + half;
+    zc[u] = gg < ng ? gi[gg] : -1;
+  }
   for (int g0 = 0; g0 < ng; g0 += 2 * kU) {
+    int zn[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int gg = g0 + 2 * kU + 2 * u + half;
+      zn[u] = gg < ng ? gi[gg] : -1;
+    }
     flt4 a0[kU], a1[kU];
-    int z[kU];
 #pragma unroll
     for (int u = 0; u < kU; ++u) {
       const int gg = g0 + 2 * u + half;
-      const bool has = gg < ng;
-      z[u] = has ? gi[gg] : -1;
       a0[u] = flt4{0.f, 0.f, 0.f, 0.f};
       a1[u] = flt4{0.f, 0.f, 0.f, 0.f};
-      if (has) {
+      if (gg < ng) {
         if (r0 < R) a0[u] = __builtin_nontemporal_load(base + gg * R + r0);
         if (two && r0 + 32 < R) a1[u] = __builtin_nontemporal_load(base + gg * R + 32 + r0);
       }
@@ -330,7 +339,7 @@ __global__ __launch_bounds__(256) void k_slab_mfma(const bwl::WorkItem* __restri
     for (int u = 0; u < kU; ++u)
 #pragma unroll
       for (int s = 0; s < 4; ++s)
-        b[u][s] = z[u] >= 0 ? XB[static_cast<int64_t>(z[u] + s) * kRhs + r0] : 0.0f;
+        b[u][s] = zc[u] >= 0 ? XB[static_cast<int64_t>(zc[u] + s) * kRhs + r0] : 0.0f;
 #pragma unroll
     for (int u = 0; u < kU; ++u)
 #pragma unroll
@@ -338,6 +347,8 @@ __global__ __launch_bounds__(256) void k_slab_mfma(const bwl::WorkItem* __restri
         acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0[u][s], b[u][s], acc0, 0, 0, 0);
         if (two) acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1[u][s], b[u][s], acc1, 0, 0, 0);
       }
+#pragma unroll
+    for (int u = 0; u < kU; ++u) zc[u] = zn[u];
   }
   const int n = lane & 31;
   auto row_of = [&](int j) { return 8 * (j >> 2) + 4 * half + (j & 3); };
@@ -370,24 +381,43 @@ __global__ __launch_bounds__(256) void k_slab_mfma(const bwl::WorkItem* __restri
 }
 
 template <int PASS>
-__global__ __launch_bounds__(64) void k_reduce_b(const bwl::Reduce* __restrict__ reds, int nred,
-                                                 const float* __restrict__ part, float* tout,
-                                                 double* Y, int64_t ldy, int nrhs,
-                                                 const int64_t* __restrict__ perm, int dof) {
+__global__ __launch_bounds__(256) void k_reduce_b(const bwl::Reduce* __restrict__ reds, int nred,
+                                                  const float* __restrict__ part, float* tout,
+                                                  double* Y, int64_t ldy, int nrhs,
+                                                  const int64_t* __restrict__ perm, int dof) {
+  // one workgroup per split slab; a thread sums four right-hand sides of one
+  // row over the parts, in part order, four parts in flight
   const int r = blockIdx.x;
   if (r >= nred) return;
   const bwl::Reduce rd = reds[r];
-  const float* base_p = part + static_cast<int64_t>(rd.part_off) * kRhs;
-  for (int e = threadIdx.x; e < rd.rows * kRhs; e += 64) {
-    float s = 0.0f;
-    for (int q = 0; q < rd.nparts; ++q) s += base_p[static_cast<int64_t>(q) * rd.rows * kRhs + e];
-    const int row = e / kRhs, nn = e % kRhs;
-    if (nn >= nrhs) continue;
-    if (PASS == 1)
-      tout[(static_cast<int64_t>(rd.out_row) + row) * kRhs + nn] = s;
-    else
-      Y[out_index<double>(static_cast<int64_t>(rd.out_row) + row, perm, dof) + nn * ldy] =
-          static_cast<double>(s);
+  const flt4* __restrict__ base_p =
+      reinterpret_cast<const flt4*>(part + static_cast<int64_t>(rd.part_off) * kRhs);
+  const int nvec = rd.rows * (kRhs / 4);
+  const int64_t stride = static_cast<int64_t>(rd.rows) * (kRhs / 4);
+  for (int e = threadIdx.x; e < nvec; e += blockDim.x) {
+    flt4 sum = {0.f, 0.f, 0.f, 0.f};
+    int q = 0;
+    for (; q + 4 <= rd.nparts; q += 4) {
+      const flt4 a = base_p[(q + 0) * stride + e];
+      const flt4 b = base_p[(q + 1) * stride + e];
+      const flt4 c = base_p[(q + 2) * stride + e];
+      const flt4 d = base_p[(q + 3) * stride + e];
+      sum += a;
+      sum += b;
+      sum += c;
+      sum += d;
+    }
+    for (; q < rd.nparts; ++q) sum += base_p[q * stride + e];
+    const int row = e / (kRhs / 4);
+    const int n0 = (e % (kRhs / 4)) * 4;
+    if (PASS == 1) {
+      *reinterpret_cast<flt4*>(tout + (static_cast<int64_t>(rd.out_row) + row) * kRhs + n0) = sum;
+    } else {
+      const int64_t o = out_index<double>(static_cast<int64_t>(rd.out_row) + row, perm, dof);
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (n0 + k < nrhs) Y[o + (n0 + k) * ldy] = static_cast<double>(sum[k]);
+    }
   }
 }
 
@@ -1193,10 +1223,10 @@ int bw_hip_matvec_batched(bw_hip_handle* hd, int nrhs, const double* X, int64_t 
                          op.p1.nwi, op.p1.gidx, static_cast<const float*>(op.p1.data), op.xb, op.xb,
                          nullptr, 0, kRhs, op.partb[0], nullptr, dof, op.p1.reds, op.p1.cnt);
     if (op.p1.nred > 0)
-      hipLaunchKernelGGL(k_reduce_b<1>, dim3(op.p1.nred), dim3(64), 0, st, op.p1.reds,
+      hipLaunchKernelGGL(k_reduce_b<1>, dim3(op.p1.nred), dim3(256), 0, st, op.p1.reds,
                          op.p1.nred, op.partb[0], op.xb, nullptr, 0, kRhs, nullptr, dof);
     if (op.p1.nbig > 0)
-      hipLaunchKernelGGL(k_reduce_b<1>, dim3(op.p1.nbig), dim3(64), 0, st, op.p1.big_reds,
+      hipLaunchKernelGGL(k_reduce_b<1>, dim3(op.p1.nbig), dim3(256), 0, st, op.p1.big_reds,
                          op.p1.nbig, op.partb[0], op.xb, nullptr, 0, kRhs, nullptr, dof);
     if (op.p2.nwi > 0)
       hipLaunchKernelGGL(k_slab_mfma<2>, dim3((op.p2.nwi + 3) / 4), dim3(256), 0, st, op.p2.wis,
@@ -1204,11 +1234,11 @@ int bw_hip_matvec_batched(bw_hip_handle* hd, int nrhs, const double* X, int64_t 
                          nullptr, Yk, ldy, nk, op.partb[1], orig ? op.perm_out : nullptr, dof,
                          op.p2.reds, op.p2.cnt);
     if (op.p2.nred > 0)
-      hipLaunchKernelGGL(k_reduce_b<2>, dim3(op.p2.nred), dim3(64), 0, st, op.p2.reds,
+      hipLaunchKernelGGL(k_reduce_b<2>, dim3(op.p2.nred), dim3(256), 0, st, op.p2.reds,
                          op.p2.nred, op.partb[1], nullptr, Yk, ldy, nk,
                          orig ? op.perm_out : nullptr, dof);
     if (op.p2.nbig > 0)
-      hipLaunchKernelGGL(k_reduce_b<2>, dim3(op.p2.nbig), dim3(64), 0, st, op.p2.big_reds,
+      hipLaunchKernelGGL(k_reduce_b<2>, dim3(op.p2.nbig), dim3(256), 0, st, op.p2.big_reds,
                          op.p2.nbig, op.partb[1], nullptr, Yk, ldy, nk,
                          orig ? op.perm_out : nullptr, dof);
     HIPCHK(hipGetLastError());

@@ -327,6 +327,11 @@ def bench_batching(args, cfg):
     rng = np.random.default_rng(1234)
     X = torch.from_numpy(rng.uniform(-1.0, 1.0, (n, k))).to("cuda:0")
     steps, warmup = args.steps, args.warmup
+    tuning = None
+    if not args.no_autotune:
+        from bigwham_dev_amd import autotune
+        tuning = autotune.tune_mfma(dev, k)
+        log(f"mfma schedules (ms per 32-RHS call): {tuning}")
     t_b32 = _timed(lambda: dev.matvec_batched(X, precision="fp32"), steps, warmup)
     x1 = X[:, 0].contiguous()
     y1 = torch.empty(n, dtype=torch.float64, device="cuda:0")
@@ -395,6 +400,7 @@ def bench_batching(args, cfg):
         "parity": {"batched_fp32_rel_err_vs_fp64": err},
         "multi_mesh": {"sides": sides, "dof": sizes, "nb_entries": e_multi, **res},
         "setup_s": {"host_build": round(t_build, 2), "multi_build": round(t_mbuild, 2)},
+        "tuning": tuning,
     }
     print(json.dumps(out), flush=True)
 

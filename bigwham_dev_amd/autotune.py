@@ -22,6 +22,7 @@ import os
 from dataclasses import dataclass, field
 
 UNROLLS = (4, 8, 16)
+MFMA_SCHEDULES = (0, 1, 2, 3)
 WI_BYTES = (48 << 10, 96 << 10, 192 << 10)
 
 
@@ -99,6 +100,29 @@ def tune_unroll(dev, precisions=None, reps: int = 5, use_cache: bool = True) -> 
     for p, u in res.unroll.items():
         dev.set_unroll(**{p: u})
     return res
+
+
+def tune_mfma(dev, nrhs: int = 32, reps: int = 5) -> dict:
+    """Time the schedules of the FP32 MFMA batched kernel (group pairs per
+    step, pipelined or not) on `nrhs` right-hand sides and keep the fastest.
+    Returns {schedule: ms per batched call, "best": schedule}."""
+    import time
+
+    import torch
+    X = torch.ones((dev.n_cols, nrhs), dtype=torch.float64, device=dev.torch_device)
+    ms = {}
+    for sch in MFMA_SCHEDULES:
+        dev.set_mfma_schedule(sch)
+        dev.matvec_batched(X, precision="fp32")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            dev.matvec_batched(X, precision="fp32")
+        torch.cuda.synchronize()
+        ms[sch] = (time.perf_counter() - t0) / reps * 1e3
+    best = min(ms, key=ms.get)
+    dev.set_mfma_schedule(best)
+    return {**ms, "best": best}
 
 
 def tune_layout(make_dev, precisions=("fp64", "fp32"), candidates=WI_BYTES, reps: int = 5):

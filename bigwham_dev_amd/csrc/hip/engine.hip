@@ -283,17 +283,94 @@ __global__ __launch_bounds__(256) void k_gather_b(const double* __restrict__ X, 
   XB[i * kRhs + c] = c < nrhs ? static_cast<float>(X[src + c * ldx]) : 0.0f;
 }
 
-template <int PASS>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_slab_mfma(const bwl::WorkItem* __restrict__ wis, int nwi,
-                                                   const int32_t* __restrict__ gidx,
-                                                   const float* __restrict__ data,
-                                                   const float* __restrict__ XB,
-                                                   float* __restrict__ tout,      // pass 1
-                                                   double* __restrict__ Y, int64_t ldy,  // pass 2
-                                                   int nrhs, float* __restrict__ part,
-                                                   const int64_t* __restrict__ perm, int dof,
-                                                   const bwl::Reduce* __restrict__ reds,
-                                                   int* __restrict__ cnt) {
+// KU group pairs per step; PIPE: the loads of step i+1 are issued before the
+// MFMAs of step i.  Addresses are clamped into the work item and invalid
+// lanes zeroed by selects, so the loop has no divergent branches.
+template <int KU, bool PIPE, bool TWO>
+__device__ __forceinline__ void mfma_stream(const flt4* __restrict__ base,
+                                            const int32_t* __restrict__ gi,
+                                            const float* __restrict__ XB, int ng, int R, int half,
+                                            int r0, floatx16& acc0, floatx16& acc1) {
+  const flt4 zero = {0.f, 0.f, 0.f, 0.f};
+  const int rA0 = min(r0, R - 1), rA1 = min(r0 + 32, R - 1);
+  const bool okA0 = r0 < R, okA1 = r0 + 32 < R;
+  auto load_z = [&](int g0, int (&z)[KU]) {
+#pragma unroll
+    for (int u = 0; u < KU; ++u) z[u] = gi[min(g0 + 2 * u + half, ng - 1)];
+  };
+  auto load_ab = [&](int g0, const int (&z)[KU], flt4 (&a0)[KU], flt4 (&a1)[KU],
+                     float (&b)[KU][4]) {
+#pragma unroll
+    for (int u = 0; u < KU; ++u) {
+      const int gg = g0 + 2 * u + half;
+      const bool ok = gg < ng;
+      const int ga = min(gg, ng - 1);
+      const flt4 v0 = __builtin_nontemporal_load(base + ga * R + rA0);
+      a0[u] = (ok && okA0) ? v0 : zero;
+      if (TWO) {
+        const flt4 v1 = __builtin_nontemporal_load(base + ga * R + rA1);
+        a1[u] = (ok && okA1) ? v1 : zero;
+      }
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const float v = XB[static_cast<int64_t>(z[u] + s) * kRhs + r0];
+        b[u][s] = ok ? v : 0.0f;
+      }
+    }
+  };
+  auto compute = [&](const flt4 (&a0)[KU], const flt4 (&a1)[KU], const float (&b)[KU][4]) {
+#pragma unroll
+    for (int u = 0; u < KU; ++u)
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0[u][s], b[u][s], acc0, 0, 0, 0);
+        if (TWO) acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1[u][s], b[u][s], acc1, 0, 0, 0);
+      }
+  };
+  int z[KU];
+  load_z(0, z);
+  if (!PIPE) {
+    for (int g0 = 0; g0 < ng; g0 += 2 * KU) {
+      int zn[KU];
+      load_z(g0 + 2 * KU, zn);
+      flt4 a0[KU], a1[KU];
+      float b[KU][4];
+      load_ab(g0, z, a0, a1, b);
+      compute(a0, a1, b);
+#pragma unroll
+      for (int u = 0; u < KU; ++u) z[u] = zn[u];
+    }
+  } else {
+    flt4 a0[KU], a1[KU];
+    float b[KU][4];
+    load_ab(0, z, a0, a1, b);
+    load_z(2 * KU, z);
+    for (int g0 = 0; g0 < ng; g0 += 2 * KU) {
+      int zn[KU];
+      load_z(g0 + 4 * KU, zn);
+      flt4 na0[KU], na1[KU];
+      float nb[KU][4];
+      load_ab(g0 + 2 * KU, z, na0, na1, nb);
+      compute(a0, a1, b);
+#pragma unroll
+      for (int u = 0; u < KU; ++u) {
+        a0[u] = na0[u];
+        if (TWO) a1[u] = na1[u];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) b[u][s] = nb[u][s];
+        z[u] = zn[u];
+      }
+    }
+  }
+}
+
+template <int PASS, int KU, bool PIPE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_slab_mfma(
+    const bwl::WorkItem* __restrict__ wis, int nwi, const int32_t* __restrict__ gidx,
+    const float* __restrict__ data, const float* __restrict__ XB,
+    float* __restrict__ tout,              // pass 1
+    double* __restrict__ Y, int64_t ldy,  // pass 2
+    int nrhs, float* __restrict__ part, const int64_t* __restrict__ perm, int dof) {
   const int lane = threadIdx.x & 63;
   const int w = xcd_remap(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
   if (w >= nwi) return;
@@ -306,49 +383,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   const int32_t* __restrict__ gi = gidx + wi.gbeg;
   floatx16 acc0 = {0}, acc1 = {0};
   const bool two = R > 32;
-  // kU group pairs per step: their A vectors and B values are loaded first,
-  // then the MFMAs run.  The input offsets of the next step are loaded one
-  // step ahead, so the B loads never wait on an offset load.
-  constexpr int kU = 4;
-  int zc[kU];
-#pragma unroll
-  for (int u = 0; u < kU; ++u) {
-    const int gg = 2 * u + half;
-    zc[u] = gg < ng ? gi[gg] : -1;
-  }
-  for (int g0 = 0; g0 < ng; g0 += 2 * kU) {
-    int zn[kU];
-#pragma unroll
-    for (int u = 0; u < kU; ++u) {
-      const int gg = g0 + 2 * kU + 2 * u + half;
-      zn[u] = gg < ng ? gi[gg] : -1;
-    }
-    flt4 a0[kU], a1[kU];
-#pragma unroll
-    for (int u = 0; u < kU; ++u) {
-      const int gg = g0 + 2 * u + half;
-      a0[u] = flt4{0.f, 0.f, 0.f, 0.f};
-      a1[u] = flt4{0.f, 0.f, 0.f, 0.f};
-      if (gg < ng) {
-        if (r0 < R) a0[u] = __builtin_nontemporal_load(base + gg * R + r0);
-        if (two && r0 + 32 < R) a1[u] = __builtin_nontemporal_load(base + gg * R + 32 + r0);
-      }
-    }
-    float b[kU][4];
-#pragma unroll
-    for (int u = 0; u < kU; ++u)
-#pragma unroll
-      for (int s = 0; s < 4; ++s)
-        b[u][s] = zc[u] >= 0 ? XB[static_cast<int64_t>(zc[u] + s) * kRhs + r0] : 0.0f;
-#pragma unroll
-    for (int u = 0; u < kU; ++u)
-#pragma unroll
-      for (int s = 0; s < 4; ++s) {
-        acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0[u][s], b[u][s], acc0, 0, 0, 0);
-        if (two) acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1[u][s], b[u][s], acc1, 0, 0, 0);
-      }
-#pragma unroll
-    for (int u = 0; u < kU; ++u) zc[u] = zn[u];
+  if (ng > 0) {
+    if (two)
+      mfma_stream<KU, PIPE, true>(base, gi, XB, ng, R, half, r0, acc0, acc1);
+    else
+      mfma_stream<KU, PIPE, false>(base, gi, XB, ng, R, half, r0, acc0, acc1);
   }
   const int n = lane & 31;
   auto row_of = [&](int j) { return 8 * (j >> 2) + 4 * half + (j & 3); };
@@ -368,6 +407,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     }
     return;
   }
+  // partial sums of split slabs are added in order by k_reduce_b
   float* pp = part + static_cast<int64_t>(wi.part_off) * kRhs;
 #pragma unroll
   for (int j = 0; j < 16; ++j) {
@@ -375,9 +415,29 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     if (row < R) pp[row * kRhs + n] = acc0[j];
     if (two && 32 + row < R) pp[(32 + row) * kRhs + n] = acc1[j];
   }
-  // partial sums of split slabs are added by k_reduce_b
-  (void)reds;
-  (void)cnt;
+}
+
+template <int PASS>
+void launch_mfma(int schedule, int blocks, hipStream_t st, const bwl::WorkItem* wis, int nwi,
+                 const int32_t* gidx, const float* data, const float* XB, float* tout, double* Y,
+                 int64_t ldy, int nrhs, float* part, const int64_t* perm, int dof) {
+  switch (schedule) {
+    case 1:
+      hipLaunchKernelGGL((k_slab_mfma<PASS, 2, true>), dim3(blocks), dim3(256), 0, st, wis, nwi,
+                         gidx, data, XB, tout, Y, ldy, nrhs, part, perm, dof);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_slab_mfma<PASS, 4, true>), dim3(blocks), dim3(256), 0, st, wis, nwi,
+                         gidx, data, XB, tout, Y, ldy, nrhs, part, perm, dof);
+      break;
+    case 3:
+      hipLaunchKernelGGL((k_slab_mfma<PASS, 8, false>), dim3(blocks), dim3(256), 0, st, wis, nwi,
+                         gidx, data, XB, tout, Y, ldy, nrhs, part, perm, dof);
+      break;
+    default:
+      hipLaunchKernelGGL((k_slab_mfma<PASS, 4, false>), dim3(blocks), dim3(256), 0, st, wis, nwi,
+                         gidx, data, XB, tout, Y, ldy, nrhs, part, perm, dof);
+  }
 }
 
 template <int PASS>
@@ -447,6 +507,7 @@ struct DevOp {
   int64_t stream_bytes = 0;
   int64_t part_elems[2] = {0, 0};
   int unroll = 16;            // loads in flight per lane in k_slab (4, 8 or 16)
+  int mfma_schedule = 0;      // k_slab_mfma variant (0-3)
   // many right-hand sides (FP32 MFMA path), allocated on first use
   float* xb = nullptr;        // xt_size x kRhs
   float* partb[2] = {nullptr, nullptr};
@@ -1219,9 +1280,9 @@ int bw_hip_matvec_batched(bw_hip_handle* hd, int nrhs, const double* X, int64_t 
                          orig ? op.perm_in : nullptr, dof);
     }
     if (op.p1.nwi > 0)
-      hipLaunchKernelGGL(k_slab_mfma<1>, dim3((op.p1.nwi + 3) / 4), dim3(256), 0, st, op.p1.wis,
-                         op.p1.nwi, op.p1.gidx, static_cast<const float*>(op.p1.data), op.xb, op.xb,
-                         nullptr, 0, kRhs, op.partb[0], nullptr, dof, op.p1.reds, op.p1.cnt);
+      launch_mfma<1>(op.mfma_schedule, (op.p1.nwi + 3) / 4, st, op.p1.wis, op.p1.nwi, op.p1.gidx,
+                     static_cast<const float*>(op.p1.data), op.xb, op.xb, nullptr, 0, kRhs,
+                     op.partb[0], nullptr, dof);
     if (op.p1.nred > 0)
       hipLaunchKernelGGL(k_reduce_b<1>, dim3(op.p1.nred), dim3(256), 0, st, op.p1.reds,
                          op.p1.nred, op.partb[0], op.xb, nullptr, 0, kRhs, nullptr, dof);
@@ -1229,10 +1290,9 @@ int bw_hip_matvec_batched(bw_hip_handle* hd, int nrhs, const double* X, int64_t 
       hipLaunchKernelGGL(k_reduce_b<1>, dim3(op.p1.nbig), dim3(256), 0, st, op.p1.big_reds,
                          op.p1.nbig, op.partb[0], op.xb, nullptr, 0, kRhs, nullptr, dof);
     if (op.p2.nwi > 0)
-      hipLaunchKernelGGL(k_slab_mfma<2>, dim3((op.p2.nwi + 3) / 4), dim3(256), 0, st, op.p2.wis,
-                         op.p2.nwi, op.p2.gidx, static_cast<const float*>(op.p2.data), op.xb,
-                         nullptr, Yk, ldy, nk, op.partb[1], orig ? op.perm_out : nullptr, dof,
-                         op.p2.reds, op.p2.cnt);
+      launch_mfma<2>(op.mfma_schedule, (op.p2.nwi + 3) / 4, st, op.p2.wis, op.p2.nwi, op.p2.gidx,
+                     static_cast<const float*>(op.p2.data), op.xb, nullptr, Yk, ldy, nk,
+                     op.partb[1], orig ? op.perm_out : nullptr, dof);
     if (op.p2.nred > 0)
       hipLaunchKernelGGL(k_reduce_b<2>, dim3(op.p2.nred), dim3(256), 0, st, op.p2.reds,
                          op.p2.nred, op.partb[1], nullptr, Yk, ldy, nk,
@@ -1277,6 +1337,13 @@ int bw_hip_set_unroll(bw_hip_handle* hd, int unroll_fp64, int unroll_fp32) {
     if (unroll_fp64) hd->op[adj][0].unroll = unroll_fp64;
     if (unroll_fp32) hd->op[adj][1].unroll = unroll_fp32;
   }
+  return BW_OK;
+}
+
+int bw_hip_set_mfma_schedule(bw_hip_handle* hd, int schedule) {
+  if (!hd) return fail(BW_EINVAL, "set_mfma_schedule: null handle");
+  if (schedule < 0 || schedule > 3) return fail(BW_EINVAL, "set_mfma_schedule: schedule must be 0-3");
+  for (int adj = 0; adj < 2; ++adj) hd->op[adj][1].mfma_schedule = schedule;
   return BW_OK;
 }
 

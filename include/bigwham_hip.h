@@ -146,6 +146,12 @@ int bw_hip_set_tuning(int64_t wi_bytes_fp64, int64_t wi_bytes_fp32);
  * 0 keeps the current value).  Chosen by bigwham_dev_amd.autotune. */
 int bw_hip_set_unroll(bw_hip_handle* hd, int unroll_fp64, int unroll_fp32);
 
+/* Schedule of the FP32 MFMA kernel of bw_hip_matvec_batched: group pairs per
+ * step and whether the next step's loads are issued before the current
+ * step's MFMAs (0: 4 pairs; 1: 2 pairs, pipelined; 2: 4 pairs, pipelined;
+ * 3: 8 pairs).  Chosen by bigwham_dev_amd.autotune. */
+int bw_hip_set_mfma_schedule(bw_hip_handle* hd, int schedule);
+
 /* Per-kernel timing.  With timing enabled every matvec records HIP events
  * between its kernels on the call's stream; bw_hip_timing_stats returns the
  * average milliseconds over the calls since bw_hip_set_timing (which resets

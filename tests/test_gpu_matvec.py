@@ -143,3 +143,18 @@ def test_autotune_unroll(gpu, cases, tmp_path, monkeypatch):
     assert all(u in autotune.UNROLLS for u in r.unroll.values())
     r2 = autotune.tune_unroll(dev, reps=2)  # from the cache
     assert r2.unroll == r.unroll and not r2.ms
+
+
+@pytest.mark.parametrize("schedule", [0, 1, 2, 3])
+def test_batched_mfma_schedules(gpu, cases, schedule):
+    """Every schedule the autotuner may pick gives the same FP32 results."""
+    import torch
+    h = cases["3DR0"]
+    print(f"mfma schedule {schedule}", flush=True)
+    dev = HipHmat(h, precisions=("fp32",))
+    dev.set_mfma_schedule(schedule)
+    X = torch.from_numpy(np.random.default_rng(13).uniform(-1, 1, (h.size, 32))).to(gpu)
+    Y = dev.matvec_batched(X, precision="fp32").cpu().numpy()
+    for j in (0, 17, 31):
+        ref = O.matvec_original(h.flat(), X[:, j].cpu().numpy())
+        assert rel(Y[:, j], ref) <= TOL32

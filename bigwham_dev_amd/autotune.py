@@ -1,14 +1,17 @@
 """Launch-schedule autotuner for the slab kernels.
 
-Two knobs shape a matvec's schedule on the device:
+Three knobs shape a matvec's schedule on the device:
 
 - the unroll depth of ``k_slab`` (loads in flight per lane: 4, 8 or 16),
   switchable per handle at no cost (``bw_hip_set_unroll``);
+- the schedule of the FP32 MFMA kernel of the batched matvec (group pairs per
+  step, loads pipelined or not; ``bw_hip_set_mfma_schedule``), also free;
 - the work-item size (bytes of factors one workgroup streams), fixed when the
   layout is built (``bw_hip_set_tuning`` before ``HipHmat``).
 
 ``tune_unroll`` times each unroll depth on the handle's own data with the
-engine's HIP-event timers and keeps the fastest one.  ``tune_layout`` rebuilds
+engine's HIP-event timers and keeps the fastest one; ``tune_mfma`` does the
+same for the MFMA schedules.  ``tune_layout`` rebuilds
 the handle for each candidate work-item size (a full re-layout and upload, so
 only worth it for matrices that are applied many times) and returns the best.
 Results are cached per matrix signature in ``~/.cache/bigwham_dev_amd/tune.json``
@@ -23,7 +26,7 @@ from dataclasses import dataclass, field
 
 UNROLLS = (4, 8, 16)
 MFMA_SCHEDULES = (0, 1, 2, 3)
-WI_BYTES = (48 << 10, 96 << 10, 192 << 10)
+WI_BYTES = (64 << 10, 96 << 10, 192 << 10, 384 << 10)
 
 
 @dataclass

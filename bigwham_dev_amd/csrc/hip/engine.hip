@@ -155,26 +155,40 @@ __global__ __launch_bounds__(256) void k_slab(const bwl::WorkItem* __restrict__ 
     nsplit = 64 / R;
     split = lane / R;
     const int row = lane - split * R;
-    if (split < nsplit) {
+    if (split < nsplit && ng > 0) {
       const VT* p = base + row;
+      const int step = U * nsplit;
+      // The input offsets of a step are loaded one step ahead, so the input
+      // entries are requested together with the factors and one memory
+      // round trip covers both (offsets past the end are clamped, unused).
+      int zc[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) zc[u] = gi[min(split + u * nsplit, ng - 1)];
       int g = split;
-      for (; g + (U - 1) * nsplit < ng; g += U * nsplit) {
+      for (; g + (U - 1) * nsplit < ng; g += step) {
         VT a[U];
-        int z[U];
+        T xv[U][V];
+        int zn[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) a[u] = __builtin_nontemporal_load(p + (g + u * nsplit) * R);
 #pragma unroll
-        for (int u = 0; u < U; ++u) z[u] = gi[g + u * nsplit];
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+          for (int v = 0; v < V; ++v) xv[u][v] = xin[zc[u] + v];
+#pragma unroll
+        for (int u = 0; u < U; ++u) zn[u] = gi[min(g + step + u * nsplit, ng - 1)];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
 #pragma unroll
           for (int v = 0; v < V; ++v) {
             if (u & 1)
-              acc1 += a[u][v] * xin[z[u] + v];
+              acc1 += a[u][v] * xv[u][v];
             else
-              acc0 += a[u][v] * xin[z[u] + v];
+              acc0 += a[u][v] * xv[u][v];
           }
         }
+#pragma unroll
+        for (int u = 0; u < U; ++u) zc[u] = zn[u];
       }
       for (; g < ng; g += nsplit) {
         const VT a = __builtin_nontemporal_load(p + g * R);

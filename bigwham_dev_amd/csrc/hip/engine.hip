@@ -121,25 +121,19 @@ __global__ __launch_bounds__(256) void k_slab(const bwl::WorkItem* __restrict__ 
   T acc0 = 0, acc1 = 0;
   int nsplit = 1;
   int split = 0;
-  // Every step loads U groups per lane; groups past the end of the work item
-  // read a clamped (valid) address and contribute zero, so short work items
-  // run the same batched loads as long ones instead of a serial tail.
-  const VT vzero = {};
   if (R > 32) {
-    // One column group per lane-step for the whole wavefront: group index
+    // One column group per iteration for the whole wavefront: group index
     // and input entries are wave-uniform (scalar loads).
-    if (lane < R && ng > 0) {
+    if (lane < R) {
       const VT* p = base + lane;
-      for (int g = 0; g < ng; g += U) {
+      int g = 0;
+      for (; g + U <= ng; g += U) {
         VT a[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-          const VT v = __builtin_nontemporal_load(p + min(g + u, ng - 1) * R);
-          a[u] = g + u < ng ? v : vzero;
-        }
+        for (int u = 0; u < U; ++u) a[u] = __builtin_nontemporal_load(p + (g + u) * R);
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-          const int z = __builtin_amdgcn_readfirstlane(gi[min(g + u, ng - 1)]);
+          const int z = __builtin_amdgcn_readfirstlane(gi[g + u]);
 #pragma unroll
           for (int v = 0; v < V; ++v) {
             if (u & 1)
@@ -149,9 +143,15 @@ __global__ __launch_bounds__(256) void k_slab(const bwl::WorkItem* __restrict__ 
           }
         }
       }
+      for (; g < ng; ++g) {
+        const VT a = __builtin_nontemporal_load(p + g * R);
+        const int z = __builtin_amdgcn_readfirstlane(gi[g]);
+#pragma unroll
+        for (int v = 0; v < V; ++v) acc0 += a[v] * xin[z + v];
+      }
     }
   } else {
-    // Several column groups per lane-step: lane = split * R + row.
+    // Several column groups per iteration: lane = split * R + row.
     nsplit = 64 / R;
     split = lane / R;
     const int row = lane - split * R;
@@ -159,26 +159,24 @@ __global__ __launch_bounds__(256) void k_slab(const bwl::WorkItem* __restrict__ 
       const VT* p = base + row;
       const int step = U * nsplit;
       // The input offsets of a step are loaded one step ahead, so the input
-      // entries are requested together with the factors.
+      // entries are requested together with the factors and one memory
+      // round trip covers both (offsets past the end are clamped, unused).
       int zc[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) zc[u] = gi[min(split + u * nsplit, ng - 1)];
-      for (int g0 = 0; g0 < ng; g0 += step) {
+      int g = split;
+      for (; g + (U - 1) * nsplit < ng; g += step) {
         VT a[U];
         T xv[U][V];
         int zn[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-          const int gg = g0 + split + u * nsplit;
-          const VT v = __builtin_nontemporal_load(p + min(gg, ng - 1) * R);
-          a[u] = gg < ng ? v : vzero;
-        }
+        for (int u = 0; u < U; ++u) a[u] = __builtin_nontemporal_load(p + (g + u * nsplit) * R);
 #pragma unroll
         for (int u = 0; u < U; ++u)
 #pragma unroll
           for (int v = 0; v < V; ++v) xv[u][v] = xin[zc[u] + v];
 #pragma unroll
-        for (int u = 0; u < U; ++u) zn[u] = gi[min(g0 + step + split + u * nsplit, ng - 1)];
+        for (int u = 0; u < U; ++u) zn[u] = gi[min(g + step + u * nsplit, ng - 1)];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
 #pragma unroll
@@ -191,6 +189,12 @@ __global__ __launch_bounds__(256) void k_slab(const bwl::WorkItem* __restrict__ 
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) zc[u] = zn[u];
+      }
+      for (; g < ng; g += nsplit) {
+        const VT a = __builtin_nontemporal_load(p + g * R);
+        const int z = gi[g];
+#pragma unroll
+        for (int v = 0; v < V; ++v) acc0 += a[v] * xin[z + v];
       }
     }
   }

@@ -419,6 +419,11 @@ def bench_adjoint(args, cfg):
     n, E = host.size, host.nb_entries()
     log(f"c5: N={n} E={E} build {t_build:.1f}s")
     dev = HipHmat(host, precisions=("fp64",), adjoint=True, device=0)
+    tuning = None
+    if not args.no_autotune:
+        from bigwham_dev_amd import autotune
+        tuning = autotune.tune_unroll(dev, use_cache=False).unroll
+        log(f"autotune {tuning}")
     xc = torch.from_numpy(host.collocation_points()).to("cuda:0")
     r2 = (xc[:, 0] ** 2 + xc[:, 1] ** 2)
     w_true = torch.exp(-r2 / (2 * 0.3 ** 2))  # Gaussian-bump opening
@@ -471,6 +476,7 @@ def bench_adjoint(args, cfg):
         "fd_check": {"directions": 20, "max_rel_err": max(errs), "median_rel_err": float(np.median(errs))},
         "adjoint_identity_rel": abs(lhs - rhs) / abs(lhs),
         "setup_s": {"host_build": round(t_build, 2)},
+        "tuning": tuning,
     }
     print(json.dumps(out), flush=True)
 

@@ -300,45 +300,43 @@ __global__ __launch_bounds__(256) void k_gather_b(const double* __restrict__ X, 
 // KU group pairs per step; PIPE: the loads of step i+1 are issued before the
 // MFMAs of step i.  Addresses are clamped into the work item and invalid
 // lanes zeroed by selects, so the loop has no divergent branches.
+// KU group pairs per step; PIPE: the loads of step i+1 are issued before the
+// MFMAs of step i.  Addresses are clamped into the work item, so every lane
+// loads valid data; a group past the end gets B = 0 (its A rows then add
+// nothing), and rows past R are never stored.  The mask is applied at the
+// MFMA, not at the load, so no VALU op waits on a load of the next step.
 template <int KU, bool PIPE, bool TWO>
 __device__ __forceinline__ void mfma_stream(const flt4* __restrict__ base,
                                             const int32_t* __restrict__ gi,
                                             const float* __restrict__ XB, int ng, int R, int half,
                                             int r0, floatx16& acc0, floatx16& acc1) {
-  const flt4 zero = {0.f, 0.f, 0.f, 0.f};
   const int rA0 = min(r0, R - 1), rA1 = min(r0 + 32, R - 1);
-  const bool okA0 = r0 < R, okA1 = r0 + 32 < R;
   auto load_z = [&](int g0, int (&z)[KU]) {
 #pragma unroll
     for (int u = 0; u < KU; ++u) z[u] = gi[min(g0 + 2 * u + half, ng - 1)];
   };
   auto load_ab = [&](int g0, const int (&z)[KU], flt4 (&a0)[KU], flt4 (&a1)[KU],
-                     float (&b)[KU][4]) {
+                     float (&b)[KU][4], bool (&ok)[KU]) {
 #pragma unroll
     for (int u = 0; u < KU; ++u) {
       const int gg = g0 + 2 * u + half;
-      const bool ok = gg < ng;
+      ok[u] = gg < ng;
       const int ga = min(gg, ng - 1);
-      const flt4 v0 = __builtin_nontemporal_load(base + ga * R + rA0);
-      a0[u] = (ok && okA0) ? v0 : zero;
-      if (TWO) {
-        const flt4 v1 = __builtin_nontemporal_load(base + ga * R + rA1);
-        a1[u] = (ok && okA1) ? v1 : zero;
-      }
+      a0[u] = __builtin_nontemporal_load(base + ga * R + rA0);
+      if (TWO) a1[u] = __builtin_nontemporal_load(base + ga * R + rA1);
 #pragma unroll
-      for (int s = 0; s < 4; ++s) {
-        const float v = XB[static_cast<int64_t>(z[u] + s) * kRhs + r0];
-        b[u][s] = ok ? v : 0.0f;
-      }
+      for (int s = 0; s < 4; ++s) b[u][s] = XB[static_cast<int64_t>(z[u] + s) * kRhs + r0];
     }
   };
-  auto compute = [&](const flt4 (&a0)[KU], const flt4 (&a1)[KU], const float (&b)[KU][4]) {
+  auto compute = [&](const flt4 (&a0)[KU], const flt4 (&a1)[KU], const float (&b)[KU][4],
+                     const bool (&ok)[KU]) {
 #pragma unroll
     for (int u = 0; u < KU; ++u)
 #pragma unroll
       for (int s = 0; s < 4; ++s) {
-        acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0[u][s], b[u][s], acc0, 0, 0, 0);
-        if (TWO) acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1[u][s], b[u][s], acc1, 0, 0, 0);
+        const float bv = ok[u] ? b[u][s] : 0.0f;
+        acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0[u][s], bv, acc0, 0, 0, 0);
+        if (TWO) acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1[u][s], bv, acc1, 0, 0, 0);
       }
   };
   int z[KU];
@@ -349,31 +347,31 @@ __device__ __forceinline__ void mfma_stream(const flt4* __restrict__ base,
       load_z(g0 + 2 * KU, zn);
       flt4 a0[KU], a1[KU];
       float b[KU][4];
-      load_ab(g0, z, a0, a1, b);
-      compute(a0, a1, b);
+      bool ok[KU];
+      load_ab(g0, z, a0, a1, b, ok);
+      compute(a0, a1, b, ok);
 #pragma unroll
       for (int u = 0; u < KU; ++u) z[u] = zn[u];
     }
   } else {
-    flt4 a0[KU], a1[KU];
-    float b[KU][4];
-    load_ab(0, z, a0, a1, b);
-    load_z(2 * KU, z);
-    for (int g0 = 0; g0 < ng; g0 += 2 * KU) {
+    // ping-pong buffers, two steps per trip, so no register copies sit
+    // between the loads of one step and the MFMAs of the other
+    flt4 pa0[KU], pa1[KU], qa0[KU], qa1[KU];
+    float pb[KU][4], qb[KU][4];
+    bool pok[KU], qok[KU];
+    load_ab(0, z, pa0, pa1, pb, pok);
+    load_z(2 * KU, z);  // offsets of step 1
+    for (int g0 = 0; g0 < ng; g0 += 4 * KU) {
       int zn[KU];
       load_z(g0 + 4 * KU, zn);
-      flt4 na0[KU], na1[KU];
-      float nb[KU][4];
-      load_ab(g0 + 2 * KU, z, na0, na1, nb);
-      compute(a0, a1, b);
-#pragma unroll
-      for (int u = 0; u < KU; ++u) {
-        a0[u] = na0[u];
-        if (TWO) a1[u] = na1[u];
-#pragma unroll
-        for (int s = 0; s < 4; ++s) b[u][s] = nb[u][s];
-        z[u] = zn[u];
-      }
+      load_ab(g0 + 2 * KU, z, qa0, qa1, qb, qok);
+      __builtin_amdgcn_sched_barrier(0);
+      compute(pa0, pa1, pb, pok);
+      if (g0 + 2 * KU >= ng) break;
+      load_z(g0 + 6 * KU, z);
+      load_ab(g0 + 4 * KU, zn, pa0, pa1, pb, pok);
+      __builtin_amdgcn_sched_barrier(0);
+      compute(qa0, qa1, qb, qok);
     }
   }
 }

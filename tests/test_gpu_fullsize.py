@@ -1,0 +1,46 @@
+"""Parity at the benchmark sizes: the 3DR0-H 200x200 square crack
+(120 000 DoF) and the 3DR0-H-mode1 500x500 crack (250 000 DoF), FP64 against
+the oracle (restatement of Hmat::matvecOriginal, src/hmat/hmatrix/hmat.cc:
+572-715) at 1e-12, FP32 at 1e-5, plus size-independent properties: the
+adjoint identity, linearity and bitwise determinism.  The 600 000-DoF 3DT0-H
+penny matrix is checked the same way by bench.py (parity block of its JSON
+line)."""
+import numpy as np
+import pytest
+
+import oracle as O
+from bigwham_dev_amd import HipHmat, HostHmat
+from bigwham_dev_amd.meshes import rect_grid
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("kernel,n", [("3DR0-H", 200), ("3DR0-H-mode1", 500)])
+def test_full_size(gpu, kernel, n):
+    import torch
+    print(f"full size {kernel} {n}x{n}", flush=True)
+    coor, conn = rect_grid(n)
+    h = HostHmat.from_mesh(kernel, coor, conn, [1.0, 0.25])
+    h.build(64, 3.0, 1e-6)
+    print(f"built N={h.size} E={h.nb_entries()}", flush=True)
+    dev = HipHmat(h, precisions=("fp64", "fp32"), adjoint=True)
+    f = h.flat()
+    rng = np.random.default_rng(1234)
+    x = rng.uniform(-1, 1, h.size)
+    ref = O.matvec_original(f, x)
+    xd = torch.from_numpy(x).to(gpu)
+    y = dev.matvec(xd)
+    assert rel(y.cpu().numpy(), ref) <= 1e-12
+    assert torch.equal(y, dev.matvec(xd))
+    assert rel(dev.matvec(xd, precision="fp32").cpu().numpy(), ref) <= 1e-5
+    w = torch.from_numpy(rng.uniform(-1, 1, h.size)).to(gpu)
+    lhs = float(torch.dot(y, w))
+    rhs = float(torch.dot(xd, dev.rmatvec(w)))
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+    y2 = dev.matvec(2.0 * xd - 3.0 * w)
+    assert float(torch.linalg.vector_norm(y2 - (2.0 * y - 3.0 * dev.matvec(w))) /
+                 torch.linalg.vector_norm(y2)) <= 1e-13

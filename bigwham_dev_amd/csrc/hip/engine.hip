@@ -107,7 +107,8 @@ __global__ __launch_bounds__(256) void k_slab(const bwl::WorkItem* __restrict__ 
                                               const T* __restrict__ data,
                                               const T* __restrict__ xin,
                                               OutT* __restrict__ out, T* __restrict__ part,
-                                              const int64_t* __restrict__ perm, int dof) {
+                                              const int64_t* __restrict__ perm, int dof,
+                                              const int32_t* __restrict__ rowmap) {
   typedef typename Vec<T>::type VT;
   constexpr int V = Vec<T>::V;
   const int lane = threadIdx.x & 63;
@@ -205,9 +206,11 @@ __global__ __launch_bounds__(256) void k_slab(const bwl::WorkItem* __restrict__ 
     if (split == 0) acc += o;
   }
   if (wi.part_off < 0) {
-    if (lane < R)
-      out[out_index<OutT>(static_cast<int64_t>(wi.out_row) + lane, perm, dof)] =
-          static_cast<OutT>(acc);
+    if (lane < R) {
+      // pass 1 rows go to their t entries through the row map
+      const int64_t orow = rowmap ? rowmap[wi.out_row + lane] : static_cast<int64_t>(wi.out_row) + lane;
+      out[out_index<OutT>(orow, perm, dof)] = static_cast<OutT>(acc);
+    }
     return;
   }
   // Split slab: the partial sums are added in work-item order by k_reduce.
@@ -220,7 +223,8 @@ __global__ __launch_bounds__(256) void k_slab(const bwl::WorkItem* __restrict__ 
 template <typename T, typename OutT>
 __global__ __launch_bounds__(64) void k_reduce(const bwl::Reduce* __restrict__ reds, int nred,
                                                const T* __restrict__ part, OutT* out,
-                                               const int64_t* __restrict__ perm, int dof) {
+                                               const int64_t* __restrict__ perm, int dof,
+                                               const int32_t* __restrict__ rowmap) {
   const int r = blockIdx.x;
   if (r >= nred) return;
   const bwl::Reduce rd = reds[r];
@@ -240,7 +244,8 @@ __global__ __launch_bounds__(64) void k_reduce(const bwl::Reduce* __restrict__ r
   }
   for (; q < rd.nparts; ++q) s0 += p[q * R];
   const T s = (s0 + s1) + (s2 + s3);
-  out[out_index<OutT>(static_cast<int64_t>(rd.out_row) + t, perm, dof)] = static_cast<OutT>(s);
+  const int64_t orow = rowmap ? rowmap[rd.out_row + t] : static_cast<int64_t>(rd.out_row) + t;
+  out[out_index<OutT>(orow, perm, dof)] = static_cast<OutT>(s);
 }
 
 // Sub-matrix selection: the input is scattered into the (zeroed) x part of xt
@@ -382,7 +387,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     const float* __restrict__ data, const float* __restrict__ XB,
     float* __restrict__ tout,              // pass 1
     double* __restrict__ Y, int64_t ldy,  // pass 2
-    int nrhs, float* __restrict__ part, const int64_t* __restrict__ perm, int dof) {
+    int nrhs, float* __restrict__ part, const int64_t* __restrict__ perm, int dof,
+    const int32_t* __restrict__ rowmap) {
   const int lane = threadIdx.x & 63;
   const int w = xcd_remap(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
   if (w >= nwi) return;
@@ -406,7 +412,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   auto store = [&](int row, float v, int64_t out_row) {
     if (row >= R || n >= nrhs) return;
     if (PASS == 1) {
-      tout[(out_row + row) * kRhs + n] = v;
+      tout[static_cast<int64_t>(rowmap[out_row + row]) * kRhs + n] = v;
     } else {
       Y[out_index<double>(out_row + row, perm, dof) + n * ldy] = static_cast<double>(v);
     }
@@ -432,23 +438,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 template <int PASS>
 void launch_mfma(int schedule, int blocks, hipStream_t st, const bwl::WorkItem* wis, int nwi,
                  const int32_t* gidx, const float* data, const float* XB, float* tout, double* Y,
-                 int64_t ldy, int nrhs, float* part, const int64_t* perm, int dof) {
+                 int64_t ldy, int nrhs, float* part, const int64_t* perm, int dof,
+                 const int32_t* rowmap) {
   switch (schedule) {
     case 1:
       hipLaunchKernelGGL((k_slab_mfma<PASS, 2, true>), dim3(blocks), dim3(256), 0, st, wis, nwi,
-                         gidx, data, XB, tout, Y, ldy, nrhs, part, perm, dof);
+                         gidx, data, XB, tout, Y, ldy, nrhs, part, perm, dof, rowmap);
       break;
     case 2:
       hipLaunchKernelGGL((k_slab_mfma<PASS, 4, true>), dim3(blocks), dim3(256), 0, st, wis, nwi,
-                         gidx, data, XB, tout, Y, ldy, nrhs, part, perm, dof);
+                         gidx, data, XB, tout, Y, ldy, nrhs, part, perm, dof, rowmap);
       break;
     case 3:
       hipLaunchKernelGGL((k_slab_mfma<PASS, 8, false>), dim3(blocks), dim3(256), 0, st, wis, nwi,
-                         gidx, data, XB, tout, Y, ldy, nrhs, part, perm, dof);
+                         gidx, data, XB, tout, Y, ldy, nrhs, part, perm, dof, rowmap);
       break;
     default:
       hipLaunchKernelGGL((k_slab_mfma<PASS, 4, false>), dim3(blocks), dim3(256), 0, st, wis, nwi,
-                         gidx, data, XB, tout, Y, ldy, nrhs, part, perm, dof);
+                         gidx, data, XB, tout, Y, ldy, nrhs, part, perm, dof, rowmap);
   }
 }
 
@@ -456,7 +463,8 @@ template <int PASS>
 __global__ __launch_bounds__(256) void k_reduce_b(const bwl::Reduce* __restrict__ reds, int nred,
                                                   const float* __restrict__ part, float* tout,
                                                   double* Y, int64_t ldy, int nrhs,
-                                                  const int64_t* __restrict__ perm, int dof) {
+                                                  const int64_t* __restrict__ perm, int dof,
+                                                  const int32_t* __restrict__ rowmap) {
   // one workgroup per split slab; a thread sums four right-hand sides of one
   // row over the parts, in part order, four parts in flight
   const int r = blockIdx.x;
@@ -483,7 +491,8 @@ __global__ __launch_bounds__(256) void k_reduce_b(const bwl::Reduce* __restrict_
     const int row = e / (kRhs / 4);
     const int n0 = (e % (kRhs / 4)) * 4;
     if (PASS == 1) {
-      *reinterpret_cast<flt4*>(tout + (static_cast<int64_t>(rd.out_row) + row) * kRhs + n0) = sum;
+      *reinterpret_cast<flt4*>(tout + static_cast<int64_t>(rowmap[rd.out_row + row]) * kRhs + n0) =
+          sum;
     } else {
       const int64_t o = out_index<double>(static_cast<int64_t>(rd.out_row) + row, perm, dof);
 #pragma unroll
@@ -498,6 +507,7 @@ __global__ __launch_bounds__(256) void k_reduce_b(const bwl::Reduce* __restrict_
 struct DevPass {
   bwl::WorkItem* wis = nullptr;
   int32_t* gidx = nullptr;
+  int32_t* rowmap = nullptr;        // pass 1: t entry of every slab row
   bwl::Reduce* reds = nullptr;      // fused reductions
   bwl::Reduce* big_reds = nullptr;  // separate reduce pass
   int* cnt = nullptr;               // per fused reduction, zero between launches
@@ -554,6 +564,12 @@ int upload_pass(const bwl::Layout& L, const bwl::Program& P, DevPass* dp, hipStr
   if ((rc = dalloc(reinterpret_cast<void**>(&dp->wis), P.wis.size() * sizeof(bwl::WorkItem))))
     return rc;
   if ((rc = dalloc(reinterpret_cast<void**>(&dp->gidx), P.gidx.size() * sizeof(int32_t)))) return rc;
+  if (!P.rowmap.empty()) {
+    if ((rc = dalloc(reinterpret_cast<void**>(&dp->rowmap), P.rowmap.size() * sizeof(int32_t))))
+      return rc;
+    HIPCHK(hipMemcpy(dp->rowmap, P.rowmap.data(), P.rowmap.size() * sizeof(int32_t),
+                     hipMemcpyHostToDevice));
+  }
   if ((rc = dalloc(reinterpret_cast<void**>(&dp->reds), P.reds.size() * sizeof(bwl::Reduce))))
     return rc;
   if ((rc = dalloc(reinterpret_cast<void**>(&dp->big_reds),
@@ -763,16 +779,16 @@ void compute_diagonal(bw_hip_handle* hd, const bw_flat_hmat* h) {
 template <typename T, typename OutT, int PASS>
 void launch_slab(int unroll, int blocks, hipStream_t st, const bwl::WorkItem* wis, int nwi,
                  const int32_t* gidx, const T* data, const T* xin, OutT* out, T* part,
-                 const int64_t* perm, int dof) {
+                 const int64_t* perm, int dof, const int32_t* rowmap) {
   if (unroll == 4)
     hipLaunchKernelGGL((k_slab<T, OutT, 4, PASS>), dim3(blocks), dim3(256), 0, st, wis, nwi, gidx,
-                       data, xin, out, part, perm, dof);
+                       data, xin, out, part, perm, dof, rowmap);
   else if (unroll == 16)
     hipLaunchKernelGGL((k_slab<T, OutT, 16, PASS>), dim3(blocks), dim3(256), 0, st, wis, nwi, gidx,
-                       data, xin, out, part, perm, dof);
+                       data, xin, out, part, perm, dof, rowmap);
   else
     hipLaunchKernelGGL((k_slab<T, OutT, 8, PASS>), dim3(blocks), dim3(256), 0, st, wis, nwi, gidx,
-                       data, xin, out, part, perm, dof);
+                       data, xin, out, part, perm, dof, rowmap);
 }
 
 template <typename T>
@@ -801,24 +817,26 @@ int launch_matvec(bw_hip_handle* hd, DevOp& op, const double* xd, double* yd, in
     const int blocks = (op.p1.nwi + 3) / 4;
     launch_slab<T, T, 1>(op.unroll, blocks, st, op.p1.wis, op.p1.nwi, op.p1.gidx,
                          static_cast<const T*>(op.p1.data), xt, xt, static_cast<T*>(op.p1.part),
-                         nullptr, dof);
+                         nullptr, dof, op.p1.rowmap);
   }
   if (hd->timing) hipEventRecord(hd->ev[2], st);
   if (op.p1.nbig > 0)
     hipLaunchKernelGGL((k_reduce<T, T>), dim3(op.p1.nbig), dim3(64), 0, st, op.p1.big_reds,
-                       op.p1.nbig, static_cast<const T*>(op.p1.part), xt, nullptr, dof);
+                       op.p1.nbig, static_cast<const T*>(op.p1.part), xt, nullptr, dof,
+                       op.p1.rowmap);
   if (hd->timing) hipEventRecord(hd->ev[3], st);
   if (op.p2.nwi > 0) {
     const int blocks = (op.p2.nwi + 3) / 4;
     launch_slab<T, double, 2>(op.unroll, blocks, st, op.p2.wis, op.p2.nwi, op.p2.gidx,
                               static_cast<const T*>(op.p2.data), xt, yd,
-                              static_cast<T*>(op.p2.part), orig ? op.perm_out : nullptr, dof);
+                              static_cast<T*>(op.p2.part), orig ? op.perm_out : nullptr, dof,
+                              nullptr);
   }
   if (hd->timing) hipEventRecord(hd->ev[4], st);
   if (op.p2.nbig > 0)
     hipLaunchKernelGGL((k_reduce<T, double>), dim3(op.p2.nbig), dim3(64), 0, st, op.p2.big_reds,
                        op.p2.nbig, static_cast<const T*>(op.p2.part), yd,
-                       orig ? op.perm_out : nullptr, dof);
+                       orig ? op.perm_out : nullptr, dof, nullptr);
   if (sd && sd->n_out > 0) {
     const int blocks = static_cast<int>((sd->n_out * dof + 255) / 256);
     hipLaunchKernelGGL(k_gather_sel, dim3(blocks), dim3(256), 0, st, yd, y_user, sd->out_dst,
@@ -1294,25 +1312,27 @@ int bw_hip_matvec_batched(bw_hip_handle* hd, int nrhs, const double* X, int64_t 
     if (op.p1.nwi > 0)
       launch_mfma<1>(op.mfma_schedule, (op.p1.nwi + 3) / 4, st, op.p1.wis, op.p1.nwi, op.p1.gidx,
                      static_cast<const float*>(op.p1.data), op.xb, op.xb, nullptr, 0, kRhs,
-                     op.partb[0], nullptr, dof);
+                     op.partb[0], nullptr, dof, op.p1.rowmap);
     if (op.p1.nred > 0)
       hipLaunchKernelGGL(k_reduce_b<1>, dim3(op.p1.nred), dim3(256), 0, st, op.p1.reds,
-                         op.p1.nred, op.partb[0], op.xb, nullptr, 0, kRhs, nullptr, dof);
+                         op.p1.nred, op.partb[0], op.xb, nullptr, 0, kRhs, nullptr, dof,
+                         op.p1.rowmap);
     if (op.p1.nbig > 0)
       hipLaunchKernelGGL(k_reduce_b<1>, dim3(op.p1.nbig), dim3(256), 0, st, op.p1.big_reds,
-                         op.p1.nbig, op.partb[0], op.xb, nullptr, 0, kRhs, nullptr, dof);
+                         op.p1.nbig, op.partb[0], op.xb, nullptr, 0, kRhs, nullptr, dof,
+                         op.p1.rowmap);
     if (op.p2.nwi > 0)
       launch_mfma<2>(op.mfma_schedule, (op.p2.nwi + 3) / 4, st, op.p2.wis, op.p2.nwi, op.p2.gidx,
                      static_cast<const float*>(op.p2.data), op.xb, nullptr, Yk, ldy, nk,
-                     op.partb[1], orig ? op.perm_out : nullptr, dof);
+                     op.partb[1], orig ? op.perm_out : nullptr, dof, nullptr);
     if (op.p2.nred > 0)
       hipLaunchKernelGGL(k_reduce_b<2>, dim3(op.p2.nred), dim3(256), 0, st, op.p2.reds,
                          op.p2.nred, op.partb[1], nullptr, Yk, ldy, nk,
-                         orig ? op.perm_out : nullptr, dof);
+                         orig ? op.perm_out : nullptr, dof, nullptr);
     if (op.p2.nbig > 0)
       hipLaunchKernelGGL(k_reduce_b<2>, dim3(op.p2.nbig), dim3(256), 0, st, op.p2.big_reds,
                          op.p2.nbig, op.partb[1], nullptr, Yk, ldy, nk,
-                         orig ? op.perm_out : nullptr, dof);
+                         orig ? op.perm_out : nullptr, dof, nullptr);
     HIPCHK(hipGetLastError());
   }
   return BW_OK;

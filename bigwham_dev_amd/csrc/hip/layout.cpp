@@ -285,24 +285,71 @@ void build_layout(const OpView& v, const LayoutOptions& opt, Layout* L) {
   const int64_t nlr = static_cast<int64_t>(v.lr_rank.size());
   const int64_t nfr = static_cast<int64_t>(v.fr_r0.size());
 
-  // ---- pass 1: t_k = W_k x_{J_k}, one program row block per low-rank block
+  // ---- pass 1: t_k = W_k x_{J_k}.  Blocks with the same column cluster
+  // share their input, so their rows are stacked into slabs of at most
+  // max_rows rows; a slab row writes its t entry through rowmap.
   {
     Program& P = L->pass1;
     P = Program{};
     P.out_rows = v.t_total;
-    std::vector<int64_t> order(nlr);
-    std::iota(order.begin(), order.end(), 0);
-    std::sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return v.toff[a] < v.toff[b]; });
+    std::vector<int64_t> order;
+    order.reserve(nlr);
+    for (int64_t k = 0; k < nlr; ++k)
+      if (v.lr_rank[k] > 0) order.push_back(k);
+    std::sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
+      if (v.lr_c0[a] != v.lr_c0[b]) return v.lr_c0[a] < v.lr_c0[b];
+      if (v.lr_c1[a] != v.lr_c1[b]) return v.lr_c1[a] < v.lr_c1[b];
+      return v.toff[a] < v.toff[b];
+    });
     RunBuilder rb{&P};
-    for (int64_t k : order) {
-      const int64_t r = v.lr_rank[k];
-      if (r == 0) continue;
-      rb.close();
-      const int32_t rbeg = static_cast<int32_t>(P.runs.size());
-      rb.add_piece(d * v.lr_c0[k], d * (v.lr_c1[k] - v.lr_c0[k]), 2, k);
-      const int32_t rend = static_cast<int32_t>(P.runs.size());
-      add_slabs(&P, r, opt.max_rows, L->xt_t + v.toff[k], 0, rbeg, rend, V);
-      // slab src rows are within W_k: record the block through the piece
+    const int64_t maxr = opt.max_rows;
+    size_t i = 0;
+    while (i < order.size()) {
+      size_t j = i;  // [i, j): one column cluster
+      while (j < order.size() && v.lr_c0[order[j]] == v.lr_c0[order[i]] &&
+             v.lr_c1[order[j]] == v.lr_c1[order[i]])
+        ++j;
+      const int64_t c0 = v.lr_c0[order[i]], ncols = d * (v.lr_c1[order[i]] - c0);
+      // chunks of at most maxr rows, packed first-fit decreasing into slabs
+      std::vector<StackPart> chunks;
+      for (size_t q = i; q < j; ++q)
+        for (int64_t r0 = 0; r0 < v.lr_rank[order[q]]; r0 += maxr)
+          chunks.push_back({order[q], static_cast<int32_t>(r0),
+                            static_cast<int32_t>(std::min(maxr, v.lr_rank[order[q]] - r0))});
+      std::stable_sort(chunks.begin(), chunks.end(),
+                       [](const StackPart& a, const StackPart& b) { return a.nrows > b.nrows; });
+      std::vector<std::vector<StackPart>> bins;
+      std::vector<int64_t> fill;
+      for (const StackPart& c : chunks) {
+        size_t b = 0;
+        while (b < bins.size() && fill[b] + c.nrows > maxr) ++b;
+        if (b == bins.size()) {
+          bins.emplace_back();
+          fill.push_back(0);
+        }
+        bins[b].push_back(c);
+        fill[b] += c.nrows;
+      }
+      for (size_t b = 0; b < bins.size(); ++b) {
+        SlabDesc sd{};
+        sd.rows = static_cast<int32_t>(fill[b]);
+        sd.out_row0 = static_cast<int64_t>(P.rowmap.size());
+        sd.stack_beg = static_cast<int32_t>(P.stack.size());
+        for (const StackPart& c : bins[b]) {
+          P.stack.push_back(c);
+          for (int32_t a = 0; a < c.nrows; ++a)
+            P.rowmap.push_back(static_cast<int32_t>(L->xt_t + v.toff[c.block] + c.row0 + a));
+        }
+        sd.stack_end = static_cast<int32_t>(P.stack.size());
+        rb.close();
+        sd.run_beg = static_cast<int32_t>(P.runs.size());
+        rb.add_piece(d * c0, ncols, 2, bins[b][0].block);
+        sd.run_end = static_cast<int32_t>(P.runs.size());
+        sd.src_row0 = 0;
+        sd.ngroups = (ncols + V - 1) / V;
+        P.slabs.push_back(sd);
+      }
+      i = j;
     }
     finalize_program(&P, opt);
   }
@@ -374,6 +421,23 @@ void fill_slabs(const Layout& L, const Program& P, int32_t s_beg, int32_t s_end,
       const RunDesc& rd = P.runs[r];
       T* runp = slab + gcur * R * V;
       int64_t col = 0;  // column within the run
+      if (sd.stack_end > sd.stack_beg) {
+        // pass 1: one run over the shared columns, rows from stacked W_k
+        int64_t rbase = 0;
+        for (int32_t q = sd.stack_beg; q < sd.stack_end; ++q) {
+          const StackPart& sp = P.stack[q];
+          int64_t rs, cs;
+          const double* src = v.source(2, sp.block, sp.row0, 0, &rs, &cs);
+          for (int64_t c = 0; c < rd.ncols; ++c) {
+            T* dst = runp + (c / V) * R * V + (c % V) + rbase * V;
+            const double* sc = src + c * cs;
+            for (int64_t a = 0; a < sp.nrows; ++a) dst[a * V] = static_cast<T>(sc[a * rs]);
+          }
+          rbase += sp.nrows;
+        }
+        gcur += (rd.ncols + V - 1) / V;
+        continue;
+      }
       for (int32_t pi = rd.piece_beg; pi < rd.piece_end; ++pi) {
         const Piece& pc = P.pieces[pi];
         int64_t row_local;

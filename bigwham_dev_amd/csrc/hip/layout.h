@@ -8,8 +8,10 @@
 // cuda_kernels_utilities.cu:5-25).  Here the same arithmetic is re-laid out
 // once, at upload, into two streaming passes over "slabs":
 //
-//   pass 1   t = B x       one slab per <=64 rows of every B_k (r_k x dof*n_k),
-//                          input = x_J, output = t_k   (t: all ranks stacked)
+//   pass 1   t = B x       the B_k of the blocks that share a column cluster J
+//                          have the same input x_J: their rows are stacked
+//                          into slabs of <=64 rows (r_k x dof*n_k each), and
+//                          each slab row writes its t entry through a row map
 //   pass 2   y = F x + A t one slab per <=64 rows of every *row interval* —
 //                          the maximal point ranges not cut by any block's
 //                          row boundary — holding side by side every F_k and
@@ -77,11 +79,19 @@ struct RunDesc {
   int32_t piece_beg, piece_end;
 };
 
+// Rows of one right factor W_k stacked into a pass-1 slab.
+struct StackPart {
+  int64_t block;
+  int32_t row0;   // first row of W_k
+  int32_t nrows;
+};
+
 struct SlabDesc {
   int32_t rows;        // R
-  int64_t out_row0;    // first output row (xt index in pass 1, y row in pass 2)
+  int64_t out_row0;    // pass 1: first entry of the slab's rows in rowmap; pass 2: first y row
   int64_t src_row0;    // first source row: within W_k (pass 1), global (pass 2)
   int32_t run_beg, run_end;
+  int32_t stack_beg, stack_end;  // pass 1: the W_k rows of the slab, in row order
   int64_t ngroups;
   int64_t data_off;
   int32_t wi_beg, wi_end;
@@ -91,6 +101,8 @@ struct SlabDesc {
 struct Program {
   std::vector<WorkItem> wis;
   std::vector<int32_t> gidx;
+  std::vector<StackPart> stack;   // pass 1: row sources of the slabs
+  std::vector<int32_t> rowmap;    // pass 1: xt index of every slab row (t entry)
   std::vector<Reduce> reds;       // fused reductions (indexed by WorkItem::red)
   std::vector<Reduce> big_reds;   // reductions left to the separate pass
   std::vector<SlabDesc> slabs;

@@ -77,7 +77,8 @@ int bw_layout_build_multi(const bw_flat_hmat* const* hs, int n_mats, int precisi
 }
 
 // info[0..1] = xt_t, xt_size; info[2+6p .. 7+6p] for pass p (0, 1) =
-// n_wi, n_gidx, n_red, data_elems, part_elems, out_rows; info[14] = V.
+// n_wi, n_gidx, n_red, data_elems, part_elems, out_rows; info[14] = V;
+// info[15] = entries of the pass-1 row map.
 int bw_layout_info(const bw_layout* lay, int64_t* info) {
   if (!lay || !info) return BW_EINVAL;
   info[0] = lay->L.xt_t;
@@ -92,6 +93,15 @@ int bw_layout_info(const bw_layout* lay, int64_t* info) {
     info[7 + 6 * p] = P[p]->out_rows;
   }
   info[14] = lay->L.opt.vec_elems;
+  info[15] = static_cast<int64_t>(lay->L.pass1.rowmap.size());
+  return BW_OK;
+}
+
+// rowmap: info[15] int32 entries, the t entry (xt index) of every pass-1 row.
+int bw_layout_export_rowmap(const bw_layout* lay, int32_t* rowmap) {
+  if (!lay || !rowmap) return BW_EINVAL;
+  const auto& m = lay->L.pass1.rowmap;
+  if (!m.empty()) std::memcpy(rowmap, m.data(), m.size() * sizeof(int32_t));
   return BW_OK;
 }
 

@@ -177,6 +177,9 @@ int bw_layout_build_multi(const bw_flat_hmat* const* hs, int n_mats, int precisi
 int bw_layout_info(const bw_layout* lay, int64_t* info /* 16 */);
 int bw_layout_export(const bw_layout* lay, int pass, int64_t* wis, int32_t* gidx,
                      int64_t* reds, double* data);
+/* The pass-1 row map (info[15] int32 entries): work item / reduce rows of
+ * pass 1 are indices into it, and it gives each row's t entry in xt. */
+int bw_layout_export_rowmap(const bw_layout* lay, int32_t* rowmap);
 void bw_layout_free(bw_layout* lay);
 const char* bw_layout_last_error(void);
 

@@ -49,6 +49,9 @@ class LayoutDump:
                                  data.ctypes.data_as(C.POINTER(C.c_double)))
             self.passes.append(dict(wis=wis[:nwi], gidx=gidx[:ngidx], reds=reds[:nred],
                                     data=data[:ndata], npart=npart, nout=nout))
+        self.rowmap = np.zeros(max(int(info[15]), 1), dtype=np.int32)
+        lib.bw_layout_export_rowmap(out, self.rowmap.ctypes.data_as(C.POINTER(C.c_int32)))
+        self.rowmap = self.rowmap[:int(info[15])].astype(np.int64)
         lib.bw_layout_free(out)
         self.dof = dof
         self.n_in = dof * (n_row if adjoint else n_col)
@@ -68,15 +71,25 @@ class LayoutDump:
                 if poff >= 0:
                     assert poff + R <= P["npart"]
                 elif p == 0:
-                    assert orow >= self.xt_t and orow + R <= self.xt_size - 64
+                    self._check_t_rows(orow, R)
                 else:
                     assert orow >= 0 and orow + R <= self.n_out
             for (orow, R, poff, npart) in P["reds"]:
                 assert poff + npart * R <= P["npart"]
                 if p == 0:
-                    assert orow >= self.xt_t and orow + R <= self.xt_size - 64
+                    self._check_t_rows(orow, R)
                 else:
                     assert 0 <= orow and orow + R <= self.n_out
+        # pass 1 writes every t entry exactly once
+        t = self.rowmap
+        assert t.size == np.unique(t).size
+        assert t.size == 0 or (t.min() >= self.xt_t and t.max() < self.xt_size - 64)
+
+    def _check_t_rows(self, orow, R):
+        """Pass-1 rows are row-map indices; their t entries lie in xt's t part."""
+        assert 0 <= orow and orow + R <= self.rowmap.size
+        t = self.rowmap[orow:orow + R]
+        assert t.min() >= self.xt_t and t.max() < self.xt_size - 64
 
     def replay(self, x: np.ndarray, perm_in=None, perm_out=None, dtype=np.float64):
         """y = H x through the layout, with the kernels' arithmetic order."""
@@ -98,13 +111,13 @@ class LayoutDump:
                 if poff >= 0:
                     part[poff:poff + R] = acc
                 elif p == 0:
-                    xt[orow:orow + R] = acc
+                    xt[self.rowmap[orow:orow + R]] = acc
                 else:
                     y[orow:orow + R] = acc
             for (orow, R, poff, npart) in P["reds"]:
                 s = part[poff:poff + npart * R].reshape(npart, R).sum(0)
                 if p == 0:
-                    xt[orow:orow + R] = s
+                    xt[self.rowmap[orow:orow + R]] = s
                 else:
                     y[orow:orow + R] = s
         if perm_out is not None:

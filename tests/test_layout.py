@@ -129,3 +129,32 @@ def test_composite_layout_replay(precision, adjoint):
     y = lay.replay(x, perm, perm)
     tol = 1e-13 if precision == "fp64" else 1e-6
     assert np.linalg.norm(y - y_ref) / np.linalg.norm(y_ref) < tol
+
+
+@pytest.mark.parametrize("case", ["single", "fr_only", "fixed_rank", "homogeneous"])
+def test_layout_edge_cases(case):
+    """Layouts of degenerate patterns replay exactly like the oracle."""
+    if case == "single":
+        coor, conn = segment_line(1)
+        h = HostHmat.from_mesh("2DS0-H", coor, conn, [1.0, 0.25])
+        h.build(32, 3.0, 1e-6)
+    elif case == "fr_only":
+        coor, conn = rect_grid(5)
+        h = HostHmat.from_mesh("3DR0-H", coor, conn, [1.0, 0.25])
+        h.build(64, 3.0, 1e-6)
+    else:
+        coor, conn = rect_grid(22)
+        h = HostHmat.from_mesh("3DR0-H", coor, conn, [1.0, 0.25],
+                               fixed_rank=3 if case == "fixed_rank" else -1,
+                               homogeneous_size_pattern=case == "homogeneous")
+        h.build(32, 3.0, 1e-6)
+    f = h.flat()
+    for precision in ("fp64", "fp32"):
+        lay = LayoutDump(f, precision, False, 1024)
+        lay.check_bounds()
+        x = np.random.default_rng(2).uniform(-1, 1, h.size)
+        tol = 1e-13 if precision == "fp64" else 1e-6
+        perm = h.permutation()
+        y = lay.replay(x, perm, perm)
+        y_ref = O.matvec_original(f, x)
+        assert np.linalg.norm(y - y_ref) <= tol * np.linalg.norm(y_ref)

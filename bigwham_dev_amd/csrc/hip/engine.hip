@@ -538,6 +538,7 @@ struct DevOp {
 // Work-item bytes for the next create; 0 = sized from the matrix (auto_wi_bytes).
 std::atomic<int64_t> g_wi_bytes64{0};
 std::atomic<int64_t> g_wi_bytes32{0};
+std::atomic<int> g_stack_rows{64};
 
 // About 32k work items: a few rounds of 8 waves on each of the 1024 SIMDs,
 // capped where a sweep of the 600k-DoF 3DT0 matrix on MI355X stopped gaining
@@ -717,6 +718,7 @@ int build_op(bw_hip_handle* hd, const bw_flat_hmat* h, bool adjoint, hipStream_t
   opt.vec_elems = Vec<T>::V;
   opt.elem_bytes = sizeof(T);
   opt.wi_bytes = sizeof(T) == 8 ? g_wi_bytes64.load() : g_wi_bytes32.load();
+  opt.stack_rows = g_stack_rows.load();
   if (opt.wi_bytes == 0) {
     int64_t E = 0;
     for (int64_t k = 0; k < h->n_fr; ++k) {
@@ -922,6 +924,12 @@ int bw_hip_set_tuning(int64_t wi64, int64_t wi32) {
   if (wi64 < 0 || wi32 < 0) return fail(BW_EINVAL, "set_tuning: negative size");
   g_wi_bytes64 = wi64;
   g_wi_bytes32 = wi32;
+  return BW_OK;
+}
+
+int bw_hip_set_stacking(int stack_rows) {
+  if (stack_rows < 0 || stack_rows > 64) return fail(BW_EINVAL, "set_stacking: rows must be 0-64");
+  g_stack_rows = stack_rows;
   return BW_OK;
 }
 

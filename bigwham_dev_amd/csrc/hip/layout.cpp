@@ -322,7 +322,10 @@ void build_layout(const OpView& v, const LayoutOptions& opt, Layout* L) {
       std::vector<int64_t> fill;
       for (const StackPart& c : chunks) {
         size_t b = 0;
-        while (b < bins.size() && fill[b] + c.nrows > maxr) ++b;
+        if (c.nrows > opt.stack_rows)
+          b = bins.size();  // a chunk this tall keeps a slab of its own
+        while (b < bins.size() && (fill[b] + c.nrows > maxr || bins[b][0].nrows > opt.stack_rows))
+          ++b;
         if (b == bins.size()) {
           bins.emplace_back();
           fill.push_back(0);

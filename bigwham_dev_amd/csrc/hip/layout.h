@@ -140,6 +140,7 @@ struct LayoutOptions {
   int elem_bytes = 8;           // 8 (FP64) or 4 (FP32)
   int max_rows = 64;            // rows per slab (<= 64)
   int64_t wi_bytes = 96 * 1024; // target bytes per work item
+  int stack_rows = 64;          // pass-1 chunks of at most this many rows share slabs
 };
 
 struct Layout {

@@ -411,5 +411,10 @@ def full_blocks(hmat, original: bool = True, rows=None, cols=None):
     return vals, ri, ci
 
 
+def set_stacking(stack_rows: int = 64) -> None:
+    """Pass-1 stacking threshold for the next HipHmat (bw_hip_set_stacking)."""
+    N.check_hip(N.hip().bw_hip_set_stacking(int(stack_rows)))
+
+
 def set_tuning(wi_bytes_fp64: int = 0, wi_bytes_fp32: int = 0) -> None:
     N.check_hip(N.hip().bw_hip_set_tuning(int(wi_bytes_fp64), int(wi_bytes_fp32)))

@@ -142,6 +142,11 @@ const char* bw_flat_full_blocks_last_error(void);
  * the matrix: about 32k work items, 8 KiB to 384 KiB FP64 / 192 KiB FP32). */
 int bw_hip_set_tuning(int64_t wi_bytes_fp64, int64_t wi_bytes_fp32);
 
+/* Pass-1 stacking for the next create: right factors whose row chunks have
+ * at most `stack_rows` rows share 64-row slabs with the other factors of the
+ * same column cluster (0: every factor keeps its own slabs; default 64). */
+int bw_hip_set_stacking(int stack_rows);
+
 /* Loads in flight per lane in the slab kernels of this handle (4, 8 or 16;
  * 0 keeps the current value).  Chosen by bigwham_dev_amd.autotune. */
 int bw_hip_set_unroll(bw_hip_handle* hd, int unroll_fp64, int unroll_fp32);

@@ -114,6 +114,8 @@ def hip() -> C.CDLL:
         lib.bw_layout_info.argtypes = [_vp, _i64]
         lib.bw_layout_export.argtypes = [_vp, C.c_int, _i64, C.POINTER(C.c_int32), _i64, _d]
         lib.bw_layout_export_rowmap.argtypes = [_vp, C.POINTER(C.c_int32)]
+        lib.bw_layout_export_xcd.argtypes = [_vp, C.c_int, C.POINTER(C.c_int32),
+                                             C.POINTER(C.c_int32)]
         lib.bw_layout_free.argtypes = [_vp]
         lib.bw_layout_free.restype = None
         lib.bw_layout_last_error.restype = C.c_char_p
@@ -171,7 +173,7 @@ HIP_SYMBOLS = ["bw_hip_create", "bw_hip_matvec", "bw_hip_rmatvec", "bw_hip_matve
                "bw_hip_set_timing",
                "bw_hip_timing_stats", "bw_hip_destroy", "bw_hip_last_error",
                "bw_layout_build", "bw_layout_info", "bw_layout_export", "bw_layout_free",
-               "bw_layout_export_rowmap",
+               "bw_layout_export_rowmap", "bw_layout_export_xcd",
                "bw_layout_last_error", "bw_layout_build_multi", "bw_hip_create_multi",
                "bw_hip_graph_create", "bw_hip_graph_launch", "bw_hip_graph_destroy",
                "bw_flat_compose", "bw_flat_composite_view", "bw_flat_composite_offsets",

@@ -66,14 +66,20 @@ struct Vec<float> {
   typedef flt4 type;
 };
 
-// Workgroups are dispatched round-robin over the 8 XCDs of the MI355X; remap
-// so that each XCD runs a contiguous range of work items (neighbouring row
-// clusters share x entries in that XCD's L2).
-__device__ __forceinline__ int xcd_remap(int bid, int nblocks) {
-  const int nx = 8;
-  const int q = nblocks / nx, r = nblocks % nx;
-  const int x = bid % nx, k = bid / nx;
-  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + k;
+// Workgroups are dispatched round-robin over the 8 XCDs of the MI355X
+// (workgroup b runs on XCD b % 8).  Each XCD gets a contiguous range of work
+// items of about equal bytes (neighbouring row clusters share x entries in
+// that XCD's L2, and no XCD is left with the tail); its workgroups take the
+// range four WIs at a time.  Returns the WI of this wave, or -1.
+struct XcdSplit {
+  int32_t s[bwl::kXcds + 1];
+};
+
+__device__ __forceinline__ int xcd_work_item(const XcdSplit& xs) {
+  const int x = blockIdx.x % bwl::kXcds;
+  const int k = blockIdx.x / bwl::kXcds;
+  const int w = xs.s[x] + 4 * k + static_cast<int>(threadIdx.x >> 6);
+  return w < xs.s[x + 1] ? w : -1;
 }
 
 template <typename T>
@@ -102,7 +108,7 @@ __device__ __forceinline__ int64_t out_index(int64_t idx, const int64_t* perm, i
 // for pass 2); U: unroll depth (independent 16-byte loads in flight per lane);
 // PASS only names the two launches apart in profiles.
 template <typename T, typename OutT, int U, int PASS>
-__global__ __launch_bounds__(256) void k_slab(const bwl::WorkItem* __restrict__ wis, int nwi,
+__global__ __launch_bounds__(256) void k_slab(const bwl::WorkItem* __restrict__ wis, XcdSplit xs,
                                               const int32_t* __restrict__ gidx,
                                               const T* __restrict__ data,
                                               const T* __restrict__ xin,
@@ -112,8 +118,8 @@ __global__ __launch_bounds__(256) void k_slab(const bwl::WorkItem* __restrict__ 
   typedef typename Vec<T>::type VT;
   constexpr int V = Vec<T>::V;
   const int lane = threadIdx.x & 63;
-  const int w = xcd_remap(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
-  if (w >= nwi) return;
+  const int w = xcd_work_item(xs);
+  if (w < 0) return;
   const bwl::WorkItem wi = wis[w];
   const int R = wi.rows;
   const int ng = wi.ngroups;
@@ -383,15 +389,15 @@ __device__ __forceinline__ void mfma_stream(const flt4* __restrict__ base,
 
 template <int PASS, int KU, bool PIPE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_slab_mfma(
-    const bwl::WorkItem* __restrict__ wis, int nwi, const int32_t* __restrict__ gidx,
+    const bwl::WorkItem* __restrict__ wis, XcdSplit xs, const int32_t* __restrict__ gidx,
     const float* __restrict__ data, const float* __restrict__ XB,
     float* __restrict__ tout,              // pass 1
     double* __restrict__ Y, int64_t ldy,  // pass 2
     int nrhs, float* __restrict__ part, const int64_t* __restrict__ perm, int dof,
     const int32_t* __restrict__ rowmap) {
   const int lane = threadIdx.x & 63;
-  const int w = xcd_remap(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
-  if (w >= nwi) return;
+  const int w = xcd_work_item(xs);
+  if (w < 0) return;
   const bwl::WorkItem wi = wis[w];
   const int R = wi.rows;
   const int ng = wi.ngroups;
@@ -436,25 +442,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 }
 
 template <int PASS>
-void launch_mfma(int schedule, int blocks, hipStream_t st, const bwl::WorkItem* wis, int nwi,
+void launch_mfma(int schedule, int blocks, hipStream_t st, const bwl::WorkItem* wis, XcdSplit xs,
                  const int32_t* gidx, const float* data, const float* XB, float* tout, double* Y,
                  int64_t ldy, int nrhs, float* part, const int64_t* perm, int dof,
                  const int32_t* rowmap) {
   switch (schedule) {
     case 1:
-      hipLaunchKernelGGL((k_slab_mfma<PASS, 2, true>), dim3(blocks), dim3(256), 0, st, wis, nwi,
+      hipLaunchKernelGGL((k_slab_mfma<PASS, 2, true>), dim3(blocks), dim3(256), 0, st, wis, xs,
                          gidx, data, XB, tout, Y, ldy, nrhs, part, perm, dof, rowmap);
       break;
     case 2:
-      hipLaunchKernelGGL((k_slab_mfma<PASS, 4, true>), dim3(blocks), dim3(256), 0, st, wis, nwi,
+      hipLaunchKernelGGL((k_slab_mfma<PASS, 4, true>), dim3(blocks), dim3(256), 0, st, wis, xs,
                          gidx, data, XB, tout, Y, ldy, nrhs, part, perm, dof, rowmap);
       break;
     case 3:
-      hipLaunchKernelGGL((k_slab_mfma<PASS, 8, false>), dim3(blocks), dim3(256), 0, st, wis, nwi,
+      hipLaunchKernelGGL((k_slab_mfma<PASS, 8, false>), dim3(blocks), dim3(256), 0, st, wis, xs,
                          gidx, data, XB, tout, Y, ldy, nrhs, part, perm, dof, rowmap);
       break;
     default:
-      hipLaunchKernelGGL((k_slab_mfma<PASS, 4, false>), dim3(blocks), dim3(256), 0, st, wis, nwi,
+      hipLaunchKernelGGL((k_slab_mfma<PASS, 4, false>), dim3(blocks), dim3(256), 0, st, wis, xs,
                          gidx, data, XB, tout, Y, ldy, nrhs, part, perm, dof, rowmap);
   }
 }
@@ -514,6 +520,8 @@ struct DevPass {
   void* data = nullptr;
   void* part = nullptr;
   int nwi = 0, nred = 0, nbig = 0;
+  XcdSplit xs = {};
+  int blocks = 0;  // workgroups of a slab launch
   int64_t data_bytes = 0;
 };
 
@@ -552,6 +560,8 @@ template <typename T>
 int upload_pass(const bwl::Layout& L, const bwl::Program& P, DevPass* dp, hipStream_t st,
                 std::vector<void*>* allocs) {
   dp->nwi = static_cast<int>(P.wis.size());
+  for (int x = 0; x <= bwl::kXcds; ++x) dp->xs.s[x] = P.xcd[x];
+  dp->blocks = P.xcd_blocks;
   dp->nred = static_cast<int>(P.reds.size());
   dp->nbig = static_cast<int>(P.big_reds.size());
   auto dalloc = [&](void** p, size_t bytes) -> int {
@@ -779,17 +789,17 @@ void compute_diagonal(bw_hip_handle* hd, const bw_flat_hmat* h) {
 }
 
 template <typename T, typename OutT, int PASS>
-void launch_slab(int unroll, int blocks, hipStream_t st, const bwl::WorkItem* wis, int nwi,
+void launch_slab(int unroll, int blocks, hipStream_t st, const bwl::WorkItem* wis, XcdSplit xs,
                  const int32_t* gidx, const T* data, const T* xin, OutT* out, T* part,
                  const int64_t* perm, int dof, const int32_t* rowmap) {
   if (unroll == 4)
-    hipLaunchKernelGGL((k_slab<T, OutT, 4, PASS>), dim3(blocks), dim3(256), 0, st, wis, nwi, gidx,
+    hipLaunchKernelGGL((k_slab<T, OutT, 4, PASS>), dim3(blocks), dim3(256), 0, st, wis, xs, gidx,
                        data, xin, out, part, perm, dof, rowmap);
   else if (unroll == 16)
-    hipLaunchKernelGGL((k_slab<T, OutT, 16, PASS>), dim3(blocks), dim3(256), 0, st, wis, nwi, gidx,
+    hipLaunchKernelGGL((k_slab<T, OutT, 16, PASS>), dim3(blocks), dim3(256), 0, st, wis, xs, gidx,
                        data, xin, out, part, perm, dof, rowmap);
   else
-    hipLaunchKernelGGL((k_slab<T, OutT, 8, PASS>), dim3(blocks), dim3(256), 0, st, wis, nwi, gidx,
+    hipLaunchKernelGGL((k_slab<T, OutT, 8, PASS>), dim3(blocks), dim3(256), 0, st, wis, xs, gidx,
                        data, xin, out, part, perm, dof, rowmap);
 }
 
@@ -816,8 +826,7 @@ int launch_matvec(bw_hip_handle* hd, DevOp& op, const double* xd, double* yd, in
   }
   if (hd->timing) hipEventRecord(hd->ev[1], st);
   if (op.p1.nwi > 0) {
-    const int blocks = (op.p1.nwi + 3) / 4;
-    launch_slab<T, T, 1>(op.unroll, blocks, st, op.p1.wis, op.p1.nwi, op.p1.gidx,
+    launch_slab<T, T, 1>(op.unroll, op.p1.blocks, st, op.p1.wis, op.p1.xs, op.p1.gidx,
                          static_cast<const T*>(op.p1.data), xt, xt, static_cast<T*>(op.p1.part),
                          nullptr, dof, op.p1.rowmap);
   }
@@ -828,8 +837,7 @@ int launch_matvec(bw_hip_handle* hd, DevOp& op, const double* xd, double* yd, in
                        op.p1.rowmap);
   if (hd->timing) hipEventRecord(hd->ev[3], st);
   if (op.p2.nwi > 0) {
-    const int blocks = (op.p2.nwi + 3) / 4;
-    launch_slab<T, double, 2>(op.unroll, blocks, st, op.p2.wis, op.p2.nwi, op.p2.gidx,
+    launch_slab<T, double, 2>(op.unroll, op.p2.blocks, st, op.p2.wis, op.p2.xs, op.p2.gidx,
                               static_cast<const T*>(op.p2.data), xt, yd,
                               static_cast<T*>(op.p2.part), orig ? op.perm_out : nullptr, dof,
                               nullptr);
@@ -1318,7 +1326,7 @@ int bw_hip_matvec_batched(bw_hip_handle* hd, int nrhs, const double* X, int64_t 
                          orig ? op.perm_in : nullptr, dof);
     }
     if (op.p1.nwi > 0)
-      launch_mfma<1>(op.mfma_schedule, (op.p1.nwi + 3) / 4, st, op.p1.wis, op.p1.nwi, op.p1.gidx,
+      launch_mfma<1>(op.mfma_schedule, op.p1.blocks, st, op.p1.wis, op.p1.xs, op.p1.gidx,
                      static_cast<const float*>(op.p1.data), op.xb, op.xb, nullptr, 0, kRhs,
                      op.partb[0], nullptr, dof, op.p1.rowmap);
     if (op.p1.nred > 0)
@@ -1330,7 +1338,7 @@ int bw_hip_matvec_batched(bw_hip_handle* hd, int nrhs, const double* X, int64_t 
                          op.p1.nbig, op.partb[0], op.xb, nullptr, 0, kRhs, nullptr, dof,
                          op.p1.rowmap);
     if (op.p2.nwi > 0)
-      launch_mfma<2>(op.mfma_schedule, (op.p2.nwi + 3) / 4, st, op.p2.wis, op.p2.nwi, op.p2.gidx,
+      launch_mfma<2>(op.mfma_schedule, op.p2.blocks, st, op.p2.wis, op.p2.xs, op.p2.gidx,
                      static_cast<const float*>(op.p2.data), op.xb, nullptr, Yk, ldy, nk,
                      op.partb[1], orig ? op.perm_out : nullptr, dof, nullptr);
     if (op.p2.nred > 0)

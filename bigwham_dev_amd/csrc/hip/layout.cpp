@@ -270,6 +270,23 @@ void finalize_program(Program* P, const LayoutOptions& opt) {
   }
   P->data_elems = data;
   P->part_elems = part;
+  // byte-balanced XCD ranges: cut the WI list where the running byte count
+  // crosses x/8 of the total (the nearer side of the crossing)
+  const int64_t nwi = static_cast<int64_t>(P->wis.size());
+  std::vector<int64_t> cum(nwi + 1, 0);
+  for (int64_t i = 0; i < nwi; ++i)
+    cum[i + 1] = cum[i] + static_cast<int64_t>(P->wis[i].ngroups) * P->wis[i].rows + 1;
+  P->xcd[0] = 0;
+  for (int x = 1; x < kXcds; ++x) {
+    const long double target = static_cast<long double>(cum[nwi]) * x / kXcds;
+    int64_t i = std::lower_bound(cum.begin(), cum.end(), static_cast<int64_t>(target)) - cum.begin();
+    if (i > 0 && target - cum[i - 1] < cum[std::min(i, nwi)] - target) --i;
+    P->xcd[x] = static_cast<int32_t>(std::max<int64_t>(P->xcd[x - 1], std::min(i, nwi)));
+  }
+  P->xcd[kXcds] = static_cast<int32_t>(nwi);
+  int32_t most = 0;
+  for (int x = 0; x < kXcds; ++x) most = std::max(most, P->xcd[x + 1] - P->xcd[x]);
+  P->xcd_blocks = kXcds * ((most + 3) / 4);
 }
 
 }  // namespace

@@ -65,6 +65,7 @@ struct Reduce {
 };
 
 constexpr int kFuseMaxParts = 0;
+constexpr int kXcds = 8;  // accelerator complex dies of an MI355X
 
 struct Piece {
   int64_t block;   // index into the view's FR or LR arrays
@@ -111,6 +112,10 @@ struct Program {
   int64_t data_elems = 0;
   int64_t part_elems = 0;
   int64_t out_rows = 0;
+  // Work items [xcd[x], xcd[x+1]) run on XCD x: eight contiguous ranges of
+  // about equal bytes; xcd_blocks workgroups (4 WIs each) cover the longest.
+  int32_t xcd[kXcds + 1] = {};
+  int32_t xcd_blocks = 0;
 };
 
 // An operator view: H itself, or H^T built from the same blocks.

@@ -134,6 +134,16 @@ int bw_layout_export(const bw_layout* lay, int pass, int64_t* wis, int32_t* gidx
   return BW_OK;
 }
 
+// xcd: 9 int32 (XCD x runs work items [xcd[x], xcd[x+1])); blocks: the
+// workgroup count of the pass's launch.
+int bw_layout_export_xcd(const bw_layout* lay, int pass, int32_t* xcd, int32_t* blocks) {
+  if (!lay || !xcd || !blocks || (pass != 1 && pass != 2)) return BW_EINVAL;
+  const bwl::Program& P = pass == 1 ? lay->L.pass1 : lay->L.pass2;
+  for (int x = 0; x <= bwl::kXcds; ++x) xcd[x] = P.xcd[x];
+  *blocks = P.xcd_blocks;
+  return BW_OK;
+}
+
 void bw_layout_free(bw_layout* lay) { delete lay; }
 
 }  // extern "C"

@@ -185,6 +185,9 @@ int bw_layout_export(const bw_layout* lay, int pass, int64_t* wis, int32_t* gidx
 /* The pass-1 row map (info[15] int32 entries): work item / reduce rows of
  * pass 1 are indices into it, and it gives each row's t entry in xt. */
 int bw_layout_export_rowmap(const bw_layout* lay, int32_t* rowmap);
+/* The split of a pass's work items over the 8 XCDs: XCD x runs work items
+ * [xcd[x], xcd[x+1]) (9 entries), with `blocks` workgroups of 4 waves. */
+int bw_layout_export_xcd(const bw_layout* lay, int pass, int32_t* xcd, int32_t* blocks);
 void bw_layout_free(bw_layout* lay);
 const char* bw_layout_last_error(void);
 

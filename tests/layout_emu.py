@@ -49,6 +49,12 @@ class LayoutDump:
                                  data.ctypes.data_as(C.POINTER(C.c_double)))
             self.passes.append(dict(wis=wis[:nwi], gidx=gidx[:ngidx], reds=reds[:nred],
                                     data=data[:ndata], npart=npart, nout=nout))
+        self.xcd = []
+        for p in (1, 2):
+            xs = np.zeros(9, dtype=np.int32)
+            nb = C.c_int32()
+            lib.bw_layout_export_xcd(out, p, xs.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(nb))
+            self.xcd.append((xs.astype(np.int64), nb.value))
         self.rowmap = np.zeros(max(int(info[15]), 1), dtype=np.int32)
         lib.bw_layout_export_rowmap(out, self.rowmap.ctypes.data_as(C.POINTER(C.c_int32)))
         self.rowmap = self.rowmap[:int(info[15])].astype(np.int64)
@@ -84,6 +90,23 @@ class LayoutDump:
         t = self.rowmap
         assert t.size == np.unique(t).size
         assert t.size == 0 or (t.min() >= self.xt_t and t.max() < self.xt_size - 64)
+
+    def check_xcd_cover(self):
+        """The launch covers every work item exactly once: XCD x's waves
+        w = xcd[x] + 4k + wave, k < blocks / 8, stop at xcd[x + 1]."""
+        for p, P in enumerate(self.passes):
+            xs, nb = self.xcd[p]
+            nwi = len(P["wis"])
+            assert xs[0] == 0 and xs[-1] == nwi and np.all(np.diff(xs) >= 0)
+            assert nb % 8 == 0 and 4 * (nb // 8) >= np.diff(xs).max(initial=0)
+            seen = np.zeros(nwi, dtype=int)
+            for b in range(nb):
+                x, k = b % 8, b // 8
+                for wave in range(4):
+                    w = xs[x] + 4 * k + wave
+                    if w < xs[x + 1]:
+                        seen[w] += 1
+            assert np.all(seen == 1)
 
     def _check_t_rows(self, orow, R):
         """Pass-1 rows are row-map indices; their t entries lie in xt's t part."""

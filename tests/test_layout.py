@@ -35,6 +35,7 @@ def test_layout_replay(name, precision, adjoint, wi_bytes):
     f = h.flat()
     lay = LayoutDump(f, precision, adjoint, wi_bytes)
     lay.check_bounds()
+    lay.check_xcd_cover()
     x = np.random.default_rng(3).uniform(-1, 1, h.size)
     y = lay.replay(x)
     y_ref = O.rmatvec(f, x) if adjoint else O.matvec(f, x)
@@ -158,3 +159,18 @@ def test_layout_edge_cases(case):
         y = lay.replay(x, perm, perm)
         y_ref = O.matvec_original(f, x)
         assert np.linalg.norm(y - y_ref) <= tol * np.linalg.norm(y_ref)
+
+
+def test_xcd_ranges_balance_bytes():
+    """Each XCD gets a contiguous range of work items of about 1/8 of the bytes."""
+    coor, conn = rect_grid(30)
+    h = HostHmat.from_mesh("3DR0-H", coor, conn, [1.0, 0.25])
+    h.build(64, 3.0, 1e-6)
+    lay = LayoutDump(h.flat(), "fp64", False, 1024)
+    for p, P in enumerate(lay.passes):
+        xs, _ = lay.xcd[p]
+        w = P["wis"]
+        assert len(w) > 64
+        b = w[:, 2] * w[:, 3] + 1
+        per = np.array([b[xs[x]:xs[x + 1]].sum() for x in range(8)])
+        assert per.max() - per.min() <= 2 * b.max()

@@ -514,12 +514,10 @@ struct DevPass {
   bwl::WorkItem* wis = nullptr;
   int32_t* gidx = nullptr;
   int32_t* rowmap = nullptr;        // pass 1: t entry of every slab row
-  bwl::Reduce* reds = nullptr;      // fused reductions
-  bwl::Reduce* big_reds = nullptr;  // separate reduce pass
-  int* cnt = nullptr;               // per fused reduction, zero between launches
+  bwl::Reduce* reds = nullptr;      // one per split slab
   void* data = nullptr;
   void* part = nullptr;
-  int nwi = 0, nred = 0, nbig = 0;
+  int nwi = 0, nred = 0;
   XcdSplit xs = {};
   int blocks = 0;  // workgroups of a slab launch
   int64_t data_bytes = 0;
@@ -563,7 +561,6 @@ int upload_pass(const bwl::Layout& L, const bwl::Program& P, DevPass* dp, hipStr
   for (int x = 0; x <= bwl::kXcds; ++x) dp->xs.s[x] = P.xcd[x];
   dp->blocks = P.xcd_blocks;
   dp->nred = static_cast<int>(P.reds.size());
-  dp->nbig = static_cast<int>(P.big_reds.size());
   auto dalloc = [&](void** p, size_t bytes) -> int {
     if (bytes == 0) bytes = 16;
     hipError_t e = hipMalloc(p, bytes);
@@ -583,11 +580,6 @@ int upload_pass(const bwl::Layout& L, const bwl::Program& P, DevPass* dp, hipStr
   }
   if ((rc = dalloc(reinterpret_cast<void**>(&dp->reds), P.reds.size() * sizeof(bwl::Reduce))))
     return rc;
-  if ((rc = dalloc(reinterpret_cast<void**>(&dp->big_reds),
-                   P.big_reds.size() * sizeof(bwl::Reduce))))
-    return rc;
-  if ((rc = dalloc(reinterpret_cast<void**>(&dp->cnt), P.reds.size() * sizeof(int)))) return rc;
-  HIPCHK(hipMemset(dp->cnt, 0, std::max<size_t>(P.reds.size(), 1) * sizeof(int)));
   if ((rc = dalloc(&dp->part, static_cast<size_t>(P.part_elems) * sizeof(T)))) return rc;
   dp->data_bytes = P.data_elems * static_cast<int64_t>(sizeof(T));
   if ((rc = dalloc(&dp->data, static_cast<size_t>(dp->data_bytes)))) return rc;
@@ -597,9 +589,6 @@ int upload_pass(const bwl::Layout& L, const bwl::Program& P, DevPass* dp, hipStr
     HIPCHK(hipMemcpy(dp->gidx, P.gidx.data(), P.gidx.size() * sizeof(int32_t), hipMemcpyHostToDevice));
   if (!P.reds.empty())
     HIPCHK(hipMemcpy(dp->reds, P.reds.data(), P.reds.size() * sizeof(bwl::Reduce), hipMemcpyHostToDevice));
-  if (!P.big_reds.empty())
-    HIPCHK(hipMemcpy(dp->big_reds, P.big_reds.data(), P.big_reds.size() * sizeof(bwl::Reduce),
-                     hipMemcpyHostToDevice));
   // data: fill slabs into pinned staging chunks and stream them up
   const int32_t nslab = static_cast<int32_t>(P.slabs.size());
   if (nslab == 0) return BW_OK;
@@ -831,9 +820,9 @@ int launch_matvec(bw_hip_handle* hd, DevOp& op, const double* xd, double* yd, in
                          nullptr, dof, op.p1.rowmap);
   }
   if (hd->timing) hipEventRecord(hd->ev[2], st);
-  if (op.p1.nbig > 0)
-    hipLaunchKernelGGL((k_reduce<T, T>), dim3(op.p1.nbig), dim3(64), 0, st, op.p1.big_reds,
-                       op.p1.nbig, static_cast<const T*>(op.p1.part), xt, nullptr, dof,
+  if (op.p1.nred > 0)
+    hipLaunchKernelGGL((k_reduce<T, T>), dim3(op.p1.nred), dim3(64), 0, st, op.p1.reds,
+                       op.p1.nred, static_cast<const T*>(op.p1.part), xt, nullptr, dof,
                        op.p1.rowmap);
   if (hd->timing) hipEventRecord(hd->ev[3], st);
   if (op.p2.nwi > 0) {
@@ -843,9 +832,9 @@ int launch_matvec(bw_hip_handle* hd, DevOp& op, const double* xd, double* yd, in
                               nullptr);
   }
   if (hd->timing) hipEventRecord(hd->ev[4], st);
-  if (op.p2.nbig > 0)
-    hipLaunchKernelGGL((k_reduce<T, double>), dim3(op.p2.nbig), dim3(64), 0, st, op.p2.big_reds,
-                       op.p2.nbig, static_cast<const T*>(op.p2.part), yd,
+  if (op.p2.nred > 0)
+    hipLaunchKernelGGL((k_reduce<T, double>), dim3(op.p2.nred), dim3(64), 0, st, op.p2.reds,
+                       op.p2.nred, static_cast<const T*>(op.p2.part), yd,
                        orig ? op.perm_out : nullptr, dof, nullptr);
   if (sd && sd->n_out > 0) {
     const int blocks = static_cast<int>((sd->n_out * dof + 255) / 256);
@@ -1333,10 +1322,6 @@ int bw_hip_matvec_batched(bw_hip_handle* hd, int nrhs, const double* X, int64_t 
       hipLaunchKernelGGL(k_reduce_b<1>, dim3(op.p1.nred), dim3(256), 0, st, op.p1.reds,
                          op.p1.nred, op.partb[0], op.xb, nullptr, 0, kRhs, nullptr, dof,
                          op.p1.rowmap);
-    if (op.p1.nbig > 0)
-      hipLaunchKernelGGL(k_reduce_b<1>, dim3(op.p1.nbig), dim3(256), 0, st, op.p1.big_reds,
-                         op.p1.nbig, op.partb[0], op.xb, nullptr, 0, kRhs, nullptr, dof,
-                         op.p1.rowmap);
     if (op.p2.nwi > 0)
       launch_mfma<2>(op.mfma_schedule, op.p2.blocks, st, op.p2.wis, op.p2.xs, op.p2.gidx,
                      static_cast<const float*>(op.p2.data), op.xb, nullptr, Yk, ldy, nk,
@@ -1344,10 +1329,6 @@ int bw_hip_matvec_batched(bw_hip_handle* hd, int nrhs, const double* X, int64_t 
     if (op.p2.nred > 0)
       hipLaunchKernelGGL(k_reduce_b<2>, dim3(op.p2.nred), dim3(256), 0, st, op.p2.reds,
                          op.p2.nred, op.partb[1], nullptr, Yk, ldy, nk,
-                         orig ? op.perm_out : nullptr, dof, nullptr);
-    if (op.p2.nbig > 0)
-      hipLaunchKernelGGL(k_reduce_b<2>, dim3(op.p2.nbig), dim3(256), 0, st, op.p2.big_reds,
-                         op.p2.nbig, op.partb[1], nullptr, Yk, ldy, nk,
                          orig ? op.perm_out : nullptr, dof, nullptr);
     HIPCHK(hipGetLastError());
   }

@@ -228,7 +228,6 @@ void finalize_program(Program* P, const LayoutOptions& opt) {
   P->wis.clear();
   P->gidx.clear();
   P->reds.clear();
-  P->big_reds.clear();
   for (size_t s = 0; s < P->slabs.size(); ++s) {
     SlabDesc& sd = P->slabs[s];
     sd.data_off = data;
@@ -244,8 +243,6 @@ void finalize_program(Program* P, const LayoutOptions& opt) {
     int64_t per = std::max<int64_t>(1, (sd.ngroups + nwi - 1) / nwi);
     nwi = std::max<int64_t>(1, (sd.ngroups + per - 1) / per);
     sd.wi_beg = static_cast<int32_t>(P->wis.size());
-    const bool fused = nwi > 1 && nwi <= kFuseMaxParts;
-    const int32_t red_idx = fused ? static_cast<int32_t>(P->reds.size()) : -1;
     for (int64_t w = 0; w < nwi; ++w) {
       const int64_t g0 = w * per;
       const int64_t g1 = std::min(sd.ngroups, g0 + per);
@@ -256,14 +253,14 @@ void finalize_program(Program* P, const LayoutOptions& opt) {
       wi.rows = sd.rows;
       wi.out_row = static_cast<int32_t>(sd.out_row0);
       wi.part_off = nwi == 1 ? -1 : static_cast<int32_t>(part + w * sd.rows);
-      wi.red = red_idx;
+      wi.pad = 0;
       P->wis.push_back(wi);
     }
     sd.wi_end = static_cast<int32_t>(P->wis.size());
     if (nwi > 1) {
       const Reduce rd{static_cast<int32_t>(sd.out_row0), sd.rows, static_cast<int32_t>(part),
                       static_cast<int32_t>(nwi)};
-      (fused ? P->reds : P->big_reds).push_back(rd);
+      P->reds.push_back(rd);
       part += nwi * sd.rows;
     }
     data += sd.ngroups * sd.rows * V;

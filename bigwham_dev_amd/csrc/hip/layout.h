@@ -48,15 +48,16 @@ struct WorkItem {
   int32_t gbeg;      // first entry in gidx
   int32_t ngroups;
   int32_t rows;      // R
-  int32_t out_row;   // output row of the WI's row 0 (direct write)
+  int32_t out_row;   // output row of the WI's row 0 (pass 1: index into rowmap)
   int32_t part_off;  // -1: direct write, else offset into the partial buffer
-  int32_t red;       // reduce entry of a split slab whose last WI sums the
-                     // partials (fused), or -1
+  int32_t pad;
 };
 
-// Reduce descriptor for a slab split over several WIs (16 bytes).  The
-// partial sums are added by a separate reduce pass after the slab pass
-// (kFuseMaxParts = 0: no slab is reduced inside the slab kernel).
+// Reduce descriptor for a slab split over several WIs (16 bytes): a separate
+// pass after the slab pass adds the partial sums in WI order.  (Reducing
+// inside the slab kernel, by its last WI, needs agent-scope release/acquire
+// fences, which on MI355X write back and invalidate the per-XCD L2 and cost
+// more than the separate launch.)
 struct Reduce {
   int32_t out_row;
   int32_t rows;
@@ -64,7 +65,6 @@ struct Reduce {
   int32_t nparts;
 };
 
-constexpr int kFuseMaxParts = 0;
 constexpr int kXcds = 8;  // accelerator complex dies of an MI355X
 
 struct Piece {
@@ -104,8 +104,7 @@ struct Program {
   std::vector<int32_t> gidx;
   std::vector<StackPart> stack;   // pass 1: row sources of the slabs
   std::vector<int32_t> rowmap;    // pass 1: xt index of every slab row (t entry)
-  std::vector<Reduce> reds;       // fused reductions (indexed by WorkItem::red)
-  std::vector<Reduce> big_reds;   // reductions left to the separate pass
+  std::vector<Reduce> reds;       // one per split slab
   std::vector<SlabDesc> slabs;
   std::vector<RunDesc> runs;
   std::vector<Piece> pieces;

@@ -87,7 +87,7 @@ int bw_layout_info(const bw_layout* lay, int64_t* info) {
   for (int p = 0; p < 2; ++p) {
     info[2 + 6 * p] = static_cast<int64_t>(P[p]->wis.size());
     info[3 + 6 * p] = static_cast<int64_t>(P[p]->gidx.size());
-    info[4 + 6 * p] = static_cast<int64_t>(P[p]->reds.size() + P[p]->big_reds.size());
+    info[4 + 6 * p] = static_cast<int64_t>(P[p]->reds.size());
     info[5 + 6 * p] = P[p]->data_elems;
     info[6 + 6 * p] = P[p]->part_elems;
     info[7 + 6 * p] = P[p]->out_rows;
@@ -105,8 +105,8 @@ int bw_layout_export_rowmap(const bw_layout* lay, int32_t* rowmap) {
   return BW_OK;
 }
 
-// wis: 8 int64 per WI (data_off, gbeg, ngroups, rows, out_row, part_off, red, 0);
-// reds: 4 int64 per reduce (fused ones first, then the separate pass's);
+// wis: 8 int64 per WI (data_off, gbeg, ngroups, rows, out_row, part_off, 0, 0);
+// reds: 4 int64 per reduce (one per split slab);
 // data in double regardless of precision.
 int bw_layout_export(const bw_layout* lay, int pass, int64_t* wis, int32_t* gidx, int64_t* reds,
                      double* data) {
@@ -117,18 +117,15 @@ int bw_layout_export(const bw_layout* lay, int pass, int64_t* wis, int32_t* gidx
       const bwl::WorkItem& w = P.wis[i];
       int64_t* o = wis + 8 * i;
       o[0] = w.data_off; o[1] = w.gbeg; o[2] = w.ngroups; o[3] = w.rows;
-      o[4] = w.out_row; o[5] = w.part_off; o[6] = w.red; o[7] = 0;
+      o[4] = w.out_row; o[5] = w.part_off; o[6] = 0; o[7] = 0;
     }
   if (gidx && !P.gidx.empty()) std::memcpy(gidx, P.gidx.data(), P.gidx.size() * sizeof(int32_t));
-  if (reds) {
-    size_t i = 0;
-    for (const auto* list : {&P.reds, &P.big_reds})
-      for (const bwl::Reduce& r : *list) {
-        reds[4 * i] = r.out_row; reds[4 * i + 1] = r.rows;
-        reds[4 * i + 2] = r.part_off; reds[4 * i + 3] = r.nparts;
-        ++i;
-      }
-  }
+  if (reds)
+    for (size_t i = 0; i < P.reds.size(); ++i) {
+      const bwl::Reduce& r = P.reds[i];
+      reds[4 * i] = r.out_row; reds[4 * i + 1] = r.rows;
+      reds[4 * i + 2] = r.part_off; reds[4 * i + 3] = r.nparts;
+    }
   if (data && !P.slabs.empty())
     bwl::fill_slabs<double>(lay->L, P, 0, static_cast<int32_t>(P.slabs.size()), data);
   return BW_OK;

@@ -10,7 +10,7 @@ Alongside the headline line it reports:
                 duration over the timed steps, against 8 TB/s;
   fp32          the same for FP32 factors with FP32 accumulation;
   gmres         time to ||b - H64 x|| / ||b|| <= 1e-8 for the FP64 path and the
-                mixed-precision path (--gmres);
+                mixed-precision path (on by default; --no-gmres skips it);
   cpu_baseline  the CPU oracle's OpenMP matvec (restatement of Hmat::matvec,
                 hmat.cc:472-537) on this host's cores, on the same H.
 """
@@ -123,7 +123,9 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--config", default=os.environ.get("BW_BENCH_CONFIG", "c3"))
-    ap.add_argument("--gmres", action="store_true", help="also time GMRES to 1e-8")
+    ap.add_argument("--gmres", action="store_true", default=True,
+                    help="also time GMRES to 1e-8, FP64 and mixed (default)")
+    ap.add_argument("--no-gmres", dest="gmres", action="store_false")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--wi-bytes", type=int, default=0)
     ap.add_argument("--copy-probe", action="store_true",

@@ -732,7 +732,9 @@ int build_op(bw_hip_handle* hd, const bw_flat_hmat* h, bool adjoint, hipStream_t
   }
   bwl::Layout L;
   bwl::build_layout(view, opt, &L);
-  int rc;
+  std::string msg;
+  int rc = bwl::check_layout_limits(L, &msg);
+  if (rc) return fail(rc, "create: " + msg);
   if ((rc = upload_pass<T>(L, L.pass1, &op.p1, st, &hd->allocs))) return rc;
   if ((rc = upload_pass<T>(L, L.pass2, &op.p2, st, &hd->allocs))) return rc;
   op.elem = sizeof(T);

@@ -2,6 +2,7 @@
 #include "layout.h"
 
 #include <algorithm>
+#include <climits>
 #include <atomic>
 #include <cmath>
 #include <cstring>
@@ -479,6 +480,23 @@ void fill_slabs(const Layout& L, const Program& P, int32_t s_beg, int32_t s_end,
 
 template void fill_slabs<double>(const Layout&, const Program&, int32_t, int32_t, double*);
 template void fill_slabs<float>(const Layout&, const Program&, int32_t, int32_t, float*);
+
+int check_layout_limits(const Layout& L, std::string* msg) {
+  const int64_t lim = INT32_MAX;
+  if (L.xt_size >= lim) {
+    *msg = "input and rank vectors too long for 32-bit offsets";
+    return BW_ESHAPE;
+  }
+  for (const Program* P : {&L.pass1, &L.pass2}) {
+    if (static_cast<int64_t>(P->gidx.size()) >= lim || P->part_elems >= lim ||
+        static_cast<int64_t>(P->rowmap.size()) >= lim ||
+        static_cast<int64_t>(P->wis.size()) >= lim / 4) {
+      *msg = "matrix too large for 32-bit layout descriptors";
+      return BW_ESHAPE;
+    }
+  }
+  return BW_OK;
+}
 
 int64_t layout_stored_entries(const Layout& L) { return L.pass1.data_elems + L.pass2.data_elems; }
 

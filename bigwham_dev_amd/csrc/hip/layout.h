@@ -168,6 +168,11 @@ void build_layout(const OpView& v, const LayoutOptions& opt, Layout* L);
 template <typename T>
 void fill_slabs(const Layout& L, const Program& P, int32_t s_beg, int32_t s_end, T* out);
 
+// The device descriptors index with 32-bit integers (group indices, input
+// and t offsets, partial-sum offsets, row map); returns BW_ESHAPE with a
+// message if a layout is too large for them.
+int check_layout_limits(const Layout& L, std::string* msg);
+
 // Total stored entries (with padding) and algorithmic entries of a layout.
 int64_t layout_stored_entries(const Layout& L);
 

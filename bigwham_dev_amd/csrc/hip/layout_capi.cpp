@@ -44,6 +44,12 @@ int build_checked(const bw_flat_hmat* h, long double area, int precision, int ad
   opt.elem_bytes = precision == BW_PREC_FP64 ? 8 : 4;
   if (wi_bytes > 0) opt.wi_bytes = wi_bytes;
   bwl::build_layout(v, opt, &lay->L);
+  rc = bwl::check_layout_limits(lay->L, &msg);
+  if (rc) {
+    g_lerr = msg;
+    delete lay;
+    return rc;
+  }
   *out = lay;
   return BW_OK;
 }

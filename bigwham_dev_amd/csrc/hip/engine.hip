@@ -294,8 +294,9 @@ __global__ __launch_bounds__(256) void k_gather_sel(const double* __restrict__ y
 constexpr int kRhs = 32;
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
+template <typename T>
 __global__ __launch_bounds__(256) void k_gather_b(const double* __restrict__ X, int64_t ldx,
-                                                  int nrhs, float* __restrict__ XB, int64_t n,
+                                                  int nrhs, T* __restrict__ XB, int64_t n,
                                                   const int64_t* __restrict__ perm, int dof) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * (blockDim.x / kRhs) + threadIdx.x / kRhs;
   const int c = threadIdx.x % kRhs;
@@ -305,7 +306,7 @@ __global__ __launch_bounds__(256) void k_gather_b(const double* __restrict__ X, 
     const int64_t p = i / dof;
     src = dof * perm[p] + (i - p * dof);
   }
-  XB[i * kRhs + c] = c < nrhs ? static_cast<float>(X[src + c * ldx]) : 0.0f;
+  XB[i * kRhs + c] = c < nrhs ? static_cast<T>(X[src + c * ldx]) : T(0);
 }
 
 // KU group pairs per step; PIPE: the loads of step i+1 are issued before the
@@ -465,46 +466,134 @@ void launch_mfma(int schedule, int blocks, hipStream_t st, const bwl::WorkItem* 
   }
 }
 
-template <int PASS>
+template <int PASS, typename T>
 __global__ __launch_bounds__(256) void k_reduce_b(const bwl::Reduce* __restrict__ reds, int nred,
-                                                  const float* __restrict__ part, float* tout,
+                                                  const T* __restrict__ part, T* tout,
                                                   double* Y, int64_t ldy, int nrhs,
                                                   const int64_t* __restrict__ perm, int dof,
                                                   const int32_t* __restrict__ rowmap) {
-  // one workgroup per split slab; a thread sums four right-hand sides of one
-  // row over the parts, in part order, four parts in flight
+  // one workgroup per split slab; a thread sums one 16-byte vector of
+  // right-hand sides of one row over the parts, in part order, four parts in
+  // flight
+  typedef typename Vec<T>::type VT;
+  constexpr int VW = Vec<T>::V;
   const int r = blockIdx.x;
   if (r >= nred) return;
   const bwl::Reduce rd = reds[r];
-  const flt4* __restrict__ base_p =
-      reinterpret_cast<const flt4*>(part + static_cast<int64_t>(rd.part_off) * kRhs);
-  const int nvec = rd.rows * (kRhs / 4);
-  const int64_t stride = static_cast<int64_t>(rd.rows) * (kRhs / 4);
+  const VT* __restrict__ base_p =
+      reinterpret_cast<const VT*>(part + static_cast<int64_t>(rd.part_off) * kRhs);
+  const int nvec = rd.rows * (kRhs / VW);
+  const int64_t stride = static_cast<int64_t>(rd.rows) * (kRhs / VW);
   for (int e = threadIdx.x; e < nvec; e += blockDim.x) {
-    flt4 sum = {0.f, 0.f, 0.f, 0.f};
+    VT sum = {};
     int q = 0;
     for (; q + 4 <= rd.nparts; q += 4) {
-      const flt4 a = base_p[(q + 0) * stride + e];
-      const flt4 b = base_p[(q + 1) * stride + e];
-      const flt4 c = base_p[(q + 2) * stride + e];
-      const flt4 d = base_p[(q + 3) * stride + e];
+      const VT a = base_p[(q + 0) * stride + e];
+      const VT b = base_p[(q + 1) * stride + e];
+      const VT c = base_p[(q + 2) * stride + e];
+      const VT d = base_p[(q + 3) * stride + e];
       sum += a;
       sum += b;
       sum += c;
       sum += d;
     }
     for (; q < rd.nparts; ++q) sum += base_p[q * stride + e];
-    const int row = e / (kRhs / 4);
-    const int n0 = (e % (kRhs / 4)) * 4;
+    const int row = e / (kRhs / VW);
+    const int n0 = (e % (kRhs / VW)) * VW;
     if (PASS == 1) {
-      *reinterpret_cast<flt4*>(tout + static_cast<int64_t>(rowmap[rd.out_row + row]) * kRhs + n0) =
+      *reinterpret_cast<VT*>(tout + static_cast<int64_t>(rowmap[rd.out_row + row]) * kRhs + n0) =
           sum;
     } else {
       const int64_t o = out_index<double>(static_cast<int64_t>(rd.out_row) + row, perm, dof);
 #pragma unroll
-      for (int k = 0; k < 4; ++k)
+      for (int k = 0; k < VW; ++k)
         if (n0 + k < nrhs) Y[o + (n0 + k) * ldy] = static_cast<double>(sum[k]);
     }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Many right-hand sides in FP64 on MFMA (v_mfma_f64_16x16x4f64), on the FP64
+// slab layout data[g][row][v] (V = 2).  One k-step covers two column groups
+// (4 columns): lane l supplies A[i][k] with i = l % 16 (row within a 16-row
+// tile) and k = l / 16, i.e. column k % 2 of group g0 + k / 2, and
+// B[k][n] = XB[z(g0 + k / 2) + k % 2][n] with n = l % 16 (+ 16 for the second
+// column tile).  Element r of a lane's accumulator is row 4 * (l / 16) + r,
+// column l % 16.  Rows past R read a clamped row and are never stored; groups
+// past the end get B = 0.
+typedef double dblx4 __attribute__((ext_vector_type(4)));
+
+template <int PASS>
+__global__ __launch_bounds__(256) void k_slab_mfma64(
+    const bwl::WorkItem* __restrict__ wis, XcdSplit xs, const int32_t* __restrict__ gidx,
+    const double* __restrict__ data, const double* __restrict__ XB,
+    double* __restrict__ tout,             // pass 1
+    double* __restrict__ Y, int64_t ldy,  // pass 2
+    int nrhs, double* __restrict__ part, const int64_t* __restrict__ perm, int dof,
+    const int32_t* __restrict__ rowmap) {
+  const int lane = threadIdx.x & 63;
+  const int w = xcd_work_item(xs);
+  if (w < 0) return;
+  const bwl::WorkItem wi = wis[w];
+  const int R = wi.rows;
+  const int ng = wi.ngroups;
+  const int k = lane >> 4, i16 = lane & 15;
+  const int gsub = k >> 1, v = k & 1;
+  const double* __restrict__ base = data + wi.data_off;
+  const int32_t* __restrict__ gi = gidx + wi.gbeg;
+  const int ntile = (R + 15) >> 4;  // 16-row tiles in use, wave-uniform
+  dblx4 acc[4][2];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) acc[t][0] = acc[t][1] = dblx4{0.0, 0.0, 0.0, 0.0};
+  int rowc[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) rowc[t] = min(16 * t + i16, R - 1);
+  constexpr int S = 2;  // k-steps (two groups each) loaded before their MFMAs
+  for (int g0 = 0; g0 < ng; g0 += 2 * S) {
+    double a[S][4], b[S][2];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      const int gg = g0 + 2 * s + gsub;
+      const bool ok = gg < ng;
+      const int ga = min(gg, ng - 1);
+      const double* __restrict__ gp = base + static_cast<int64_t>(ga) * R * 2 + v;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) a[s][t] = t < ntile ? gp[rowc[t] * 2] : 0.0;
+      const int64_t zr = static_cast<int64_t>(gi[ga] + v) * kRhs;
+      const double b0 = XB[zr + i16], b1 = XB[zr + 16 + i16];
+      b[s][0] = ok ? b0 : 0.0;
+      b[s][1] = ok ? b1 : 0.0;
+    }
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+        if (t < ntile) {
+          acc[t][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s][t], b[s][0], acc[t][0], 0, 0, 0);
+          acc[t][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s][t], b[s][1], acc[t][1], 0, 0, 0);
+        }
+  }
+  double* pp = wi.part_off >= 0 ? part + static_cast<int64_t>(wi.part_off) * kRhs : nullptr;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    if (t >= ntile) break;
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = 16 * t + 4 * k + r;
+        const int n = 16 * c + i16;
+        if (row >= R) continue;
+        const double val = acc[t][c][r];
+        if (pp) {
+          pp[row * kRhs + n] = val;  // partial sums, added in order by k_reduce_b
+        } else if (n < nrhs) {
+          if (PASS == 1)
+            tout[static_cast<int64_t>(rowmap[wi.out_row + row]) * kRhs + n] = val;
+          else
+            Y[out_index<double>(static_cast<int64_t>(wi.out_row) + row, perm, dof) + n * ldy] = val;
+        }
+      }
   }
 }
 
@@ -537,8 +626,8 @@ struct DevOp {
   int unroll = 16;            // loads in flight per lane in k_slab (4, 8 or 16)
   int mfma_schedule = 0;      // k_slab_mfma variant (0-3)
   // many right-hand sides (FP32 MFMA path), allocated on first use
-  float* xb = nullptr;        // xt_size x kRhs
-  float* partb[2] = {nullptr, nullptr};
+  void* xb = nullptr;        // xt_size x kRhs
+  void* partb[2] = {nullptr, nullptr};
 };
 
 // Work-item bytes for the next create; 0 = sized from the matrix (auto_wi_bytes).
@@ -1285,22 +1374,16 @@ int bw_hip_matvec_batched(bw_hip_handle* hd, int nrhs, const double* X, int64_t 
   DevOp& op = hd->op[0][precision == BW_PREC_FP32 ? 1 : 0];
   if (!op.present) return fail(BW_ENOTSUP, "matvec_batched: precision not built");
   if (ldx < op.n_in || ldy < op.n_out) return fail(BW_ESHAPE, "matvec_batched: leading dimension too small");
-  if (precision == BW_PREC_FP64) {
-    // FP64: the streaming path, one right-hand side at a time
-    for (int k = 0; k < nrhs; ++k) {
-      int rc = do_matvec(hd, false, X + k * ldx, Y + k * ldy, ordering, precision, 1, stream);
-      if (rc) return rc;
-    }
-    return BW_OK;
-  }
   HIPCHK(hipSetDevice(hd->device));
   hipStream_t st = static_cast<hipStream_t>(stream);
+  const bool fp64 = precision == BW_PREC_FP64;
+  const size_t es = fp64 ? sizeof(double) : sizeof(float);
   if (!op.xb) {
-    HIPCHK(hipMalloc(&op.xb, static_cast<size_t>(op.xt_size) * kRhs * sizeof(float)));
+    HIPCHK(hipMalloc(&op.xb, static_cast<size_t>(op.xt_size) * kRhs * es));
     hd->allocs.push_back(op.xb);
-    HIPCHK(hipMemset(op.xb, 0, static_cast<size_t>(op.xt_size) * kRhs * sizeof(float)));
+    HIPCHK(hipMemset(op.xb, 0, static_cast<size_t>(op.xt_size) * kRhs * es));
     for (int p = 0; p < 2; ++p) {
-      HIPCHK(hipMalloc(&op.partb[p], std::max<int64_t>(op.part_elems[p], 1) * kRhs * sizeof(float)));
+      HIPCHK(hipMalloc(&op.partb[p], std::max<int64_t>(op.part_elems[p], 1) * kRhs * es));
       hd->allocs.push_back(op.partb[p]);
     }
   }
@@ -1310,28 +1393,51 @@ int bw_hip_matvec_batched(bw_hip_handle* hd, int nrhs, const double* X, int64_t 
     const int nk = std::min(kRhs, nrhs - k0);
     const double* Xk = X + static_cast<int64_t>(k0) * ldx;
     double* Yk = Y + static_cast<int64_t>(k0) * ldy;
-    {
-      const int64_t rows_per_block = 256 / kRhs;
-      const int blocks = static_cast<int>((op.n_in + rows_per_block - 1) / rows_per_block);
-      hipLaunchKernelGGL(k_gather_b, dim3(blocks), dim3(256), 0, st, Xk, ldx, nk, op.xb, op.n_in,
-                         orig ? op.perm_in : nullptr, dof);
+    const int64_t rows_per_block = 256 / kRhs;
+    const int gblocks = static_cast<int>((op.n_in + rows_per_block - 1) / rows_per_block);
+    const int64_t* pin = orig ? op.perm_in : nullptr;
+    const int64_t* pout = orig ? op.perm_out : nullptr;
+    if (fp64) {
+      double* xb = static_cast<double*>(op.xb);
+      double* pb0 = static_cast<double*>(op.partb[0]);
+      double* pb1 = static_cast<double*>(op.partb[1]);
+      hipLaunchKernelGGL(k_gather_b<double>, dim3(gblocks), dim3(256), 0, st, Xk, ldx, nk, xb,
+                         op.n_in, pin, dof);
+      if (op.p1.nwi > 0)
+        hipLaunchKernelGGL(k_slab_mfma64<1>, dim3(op.p1.blocks), dim3(256), 0, st, op.p1.wis,
+                           op.p1.xs, op.p1.gidx, static_cast<const double*>(op.p1.data), xb, xb,
+                           nullptr, 0, kRhs, pb0, nullptr, dof, op.p1.rowmap);
+      if (op.p1.nred > 0)
+        hipLaunchKernelGGL((k_reduce_b<1, double>), dim3(op.p1.nred), dim3(256), 0, st, op.p1.reds,
+                           op.p1.nred, pb0, xb, nullptr, 0, kRhs, nullptr, dof, op.p1.rowmap);
+      if (op.p2.nwi > 0)
+        hipLaunchKernelGGL(k_slab_mfma64<2>, dim3(op.p2.blocks), dim3(256), 0, st, op.p2.wis,
+                           op.p2.xs, op.p2.gidx, static_cast<const double*>(op.p2.data), xb,
+                           nullptr, Yk, ldy, nk, pb1, pout, dof, nullptr);
+      if (op.p2.nred > 0)
+        hipLaunchKernelGGL((k_reduce_b<2, double>), dim3(op.p2.nred), dim3(256), 0, st, op.p2.reds,
+                           op.p2.nred, pb1, nullptr, Yk, ldy, nk, pout, dof, nullptr);
+    } else {
+      float* xb = static_cast<float*>(op.xb);
+      float* pb0 = static_cast<float*>(op.partb[0]);
+      float* pb1 = static_cast<float*>(op.partb[1]);
+      hipLaunchKernelGGL(k_gather_b<float>, dim3(gblocks), dim3(256), 0, st, Xk, ldx, nk, xb,
+                         op.n_in, pin, dof);
+      if (op.p1.nwi > 0)
+        launch_mfma<1>(op.mfma_schedule, op.p1.blocks, st, op.p1.wis, op.p1.xs, op.p1.gidx,
+                       static_cast<const float*>(op.p1.data), xb, xb, nullptr, 0, kRhs, pb0,
+                       nullptr, dof, op.p1.rowmap);
+      if (op.p1.nred > 0)
+        hipLaunchKernelGGL((k_reduce_b<1, float>), dim3(op.p1.nred), dim3(256), 0, st, op.p1.reds,
+                           op.p1.nred, pb0, xb, nullptr, 0, kRhs, nullptr, dof, op.p1.rowmap);
+      if (op.p2.nwi > 0)
+        launch_mfma<2>(op.mfma_schedule, op.p2.blocks, st, op.p2.wis, op.p2.xs, op.p2.gidx,
+                       static_cast<const float*>(op.p2.data), xb, nullptr, Yk, ldy, nk, pb1, pout,
+                       dof, nullptr);
+      if (op.p2.nred > 0)
+        hipLaunchKernelGGL((k_reduce_b<2, float>), dim3(op.p2.nred), dim3(256), 0, st, op.p2.reds,
+                           op.p2.nred, pb1, nullptr, Yk, ldy, nk, pout, dof, nullptr);
     }
-    if (op.p1.nwi > 0)
-      launch_mfma<1>(op.mfma_schedule, op.p1.blocks, st, op.p1.wis, op.p1.xs, op.p1.gidx,
-                     static_cast<const float*>(op.p1.data), op.xb, op.xb, nullptr, 0, kRhs,
-                     op.partb[0], nullptr, dof, op.p1.rowmap);
-    if (op.p1.nred > 0)
-      hipLaunchKernelGGL(k_reduce_b<1>, dim3(op.p1.nred), dim3(256), 0, st, op.p1.reds,
-                         op.p1.nred, op.partb[0], op.xb, nullptr, 0, kRhs, nullptr, dof,
-                         op.p1.rowmap);
-    if (op.p2.nwi > 0)
-      launch_mfma<2>(op.mfma_schedule, op.p2.blocks, st, op.p2.wis, op.p2.xs, op.p2.gidx,
-                     static_cast<const float*>(op.p2.data), op.xb, nullptr, Yk, ldy, nk,
-                     op.partb[1], orig ? op.perm_out : nullptr, dof, nullptr);
-    if (op.p2.nred > 0)
-      hipLaunchKernelGGL(k_reduce_b<2>, dim3(op.p2.nred), dim3(256), 0, st, op.p2.reds,
-                         op.p2.nred, op.partb[1], nullptr, Yk, ldy, nk,
-                         orig ? op.perm_out : nullptr, dof, nullptr);
     HIPCHK(hipGetLastError());
   }
   return BW_OK;

@@ -89,7 +89,9 @@ int bw_hip_rmatvec(bw_hip_handle* hd, const double* x, double* y, int ordering,
 
 /* Y = H X for nrhs right-hand sides.  X is (n_cols x nrhs) column-major with
  * leading dimension ldx, Y (n_rows x nrhs) with ldy; device pointers only.
- * FP32 runs the pass on MFMA thin-GEMM tiles; FP64 on the streaming path. */
+ * Both precisions run the passes as thin GEMMs on MFMA, 32 right-hand sides
+ * per tile (FP32: v_mfma_f32_32x32x2f32; FP64: v_mfma_f64_16x16x4f64, same
+ * summation bar as the single-RHS FP64 path). */
 int bw_hip_matvec_batched(bw_hip_handle* hd, int nrhs, const double* X, int64_t ldx,
                           double* Y, int64_t ldy, int ordering, int precision,
                           void* stream);

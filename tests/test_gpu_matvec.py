@@ -88,15 +88,25 @@ def test_diagonal(gpu, cases):
     assert HipHmat(g).diagonal().sum() == g.size
 
 
-def test_batched(gpu, cases):
+@pytest.mark.parametrize("nrhs", [1, 8, 32, 37])
+@pytest.mark.parametrize("original", [False, True])
+def test_batched_fp64_mfma(gpu, cases, nrhs, original):
+    """Many right-hand sides in FP64 on MFMA (v_mfma_f64_16x16x4f64) against
+    the oracle column by column, at the FP64 bar (1e-12)."""
     import torch
-    h = cases["3DR0"]
-    dev = HipHmat(h, precisions=("fp64", "fp32"))
-    rng = np.random.default_rng(5)
-    X = rng.uniform(-1, 1, (h.size, 8))
-    Y = dev.matvec_batched(torch.from_numpy(X).to(gpu)).cpu().numpy()
-    for k in range(8):
-        assert rel(Y[:, k], O.matvec_original(h.flat(), X[:, k])) <= TOL64
+    for name in ("3DR0", "2DS0", "3DR0m1", "gaussian"):
+        h = cases[name]
+        print(f"fp64 batched case {name} nrhs={nrhs} original={original}", flush=True)
+        dev = HipHmat(h, precisions=("fp64", "fp32"))
+        rng = np.random.default_rng(100 + nrhs)
+        X = rng.uniform(-1, 1, (h.size, nrhs))
+        Y = dev.matvec_batched(torch.from_numpy(X).to(gpu), original=original).cpu().numpy()
+        fn = O.matvec_original if original else O.matvec
+        for k in range(nrhs):
+            assert rel(Y[:, k], fn(h.flat(), X[:, k])) <= TOL64, (name, k)
+        # two calls in a row give the same bits
+        Y2 = dev.matvec_batched(torch.from_numpy(X).to(gpu), original=original).cpu().numpy()
+        assert np.array_equal(Y, Y2)
 
 
 @pytest.mark.parametrize("nrhs", [5, 32, 40])

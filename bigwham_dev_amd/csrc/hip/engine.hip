@@ -518,8 +518,8 @@ __global__ __launch_bounds__(256) void k_reduce_b(const bwl::Reduce* __restrict_
 // (4 columns): lane l supplies A[i][k] with i = l % 16 (row within a 16-row
 // tile) and k = l / 16, i.e. column k % 2 of group g0 + k / 2, and
 // B[k][n] = XB[z(g0 + k / 2) + k % 2][n] with n = l % 16 (+ 16 for the second
-// column tile).  Element r of a lane's accumulator is row 4 * (l / 16) + r,
-// column l % 16.  Rows past R read a clamped row and are never stored; groups
+// column tile).  Element r of a lane's accumulator is row l / 16 + 4 r,
+// column l % 16 (measured on gfx950 with one-hot operands).  Rows past R read a clamped row and are never stored; groups
 // past the end get B = 0.
 typedef double dblx4 __attribute__((ext_vector_type(4)));
 
@@ -581,7 +581,7 @@ __global__ __launch_bounds__(256) void k_slab_mfma64(
     for (int c = 0; c < 2; ++c)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = 16 * t + 4 * k + r;
+        const int row = 16 * t + k + 4 * r;
         const int n = 16 * c + i16;
         if (row >= R) continue;
         const double val = acc[t][c][r];

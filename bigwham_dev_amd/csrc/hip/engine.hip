@@ -524,7 +524,7 @@ __global__ __launch_bounds__(256) void k_reduce_b(const bwl::Reduce* __restrict_
 typedef double dblx4 __attribute__((ext_vector_type(4)));
 
 template <int PASS>
-__global__ __launch_bounds__(256) void k_slab_mfma64(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_slab_mfma64(
     const bwl::WorkItem* __restrict__ wis, XcdSplit xs, const int32_t* __restrict__ gidx,
     const double* __restrict__ data, const double* __restrict__ XB,
     double* __restrict__ tout,             // pass 1

@@ -16,17 +16,15 @@ class HMatVec(torch.autograd.Function):
         ctx.hmat, ctx.precision, ctx.original = hmat, precision, original
         if x.dim() == 1:
             return hmat.matvec(x.contiguous(), original=original, precision=precision)
-        return torch.stack([hmat.matvec(x[:, k].contiguous(), original=original,
-                                        precision=precision) for k in range(x.shape[1])], 1)
+        return hmat.matvec_batched(x, original=original, precision=precision)
 
     @staticmethod
     def backward(ctx, gy):
         h = ctx.hmat
         if gy.dim() == 1:
             gx = h.rmatvec(gy.contiguous(), original=ctx.original, precision=ctx.precision)
-        else:
-            gx = torch.stack([h.rmatvec(gy[:, k].contiguous(), original=ctx.original,
-                                        precision=ctx.precision) for k in range(gy.shape[1])], 1)
+        else:  # many columns: one MFMA-tiled H^T call
+            gx = h.rmatvec_batched(gy, original=ctx.original, precision=ctx.precision)
         return gx, None, None, None
 
 

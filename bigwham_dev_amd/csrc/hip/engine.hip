@@ -1363,16 +1363,22 @@ int bw_hip_rmatvec(bw_hip_handle* hd, const double* x, double* y, int ordering, 
   return do_matvec(hd, true, x, y, ordering, precision, on_device, stream);
 }
 
-int bw_hip_matvec_batched(bw_hip_handle* hd, int nrhs, const double* X, int64_t ldx, double* Y,
-                          int64_t ldy, int ordering, int precision, void* stream) {
+}  // extern "C"
+
+namespace {
+
+int batched_impl(bw_hip_handle* hd, bool adjoint, int nrhs, const double* X, int64_t ldx,
+                 double* Y, int64_t ldy, int ordering, int precision, void* stream) {
   if (!hd || !X || !Y || nrhs <= 0) return fail(BW_EINVAL, "matvec_batched: bad argument");
   if (hd->sel.on) return fail(BW_ENOTSUP, "matvec_batched: not available for a selection");
   if (ordering != BW_ORDER_PERMUTED && ordering != BW_ORDER_ORIGINAL)
     return fail(BW_EINVAL, "matvec_batched: bad ordering");
   if (precision != BW_PREC_FP64 && precision != BW_PREC_FP32)
     return fail(BW_EINVAL, "matvec_batched: bad precision");
-  DevOp& op = hd->op[0][precision == BW_PREC_FP32 ? 1 : 0];
-  if (!op.present) return fail(BW_ENOTSUP, "matvec_batched: precision not built");
+  DevOp& op = hd->op[adjoint ? 1 : 0][precision == BW_PREC_FP32 ? 1 : 0];
+  if (!op.present)
+    return fail(BW_ENOTSUP, adjoint ? "rmatvec_batched: adjoint / precision not built"
+                                    : "matvec_batched: precision not built");
   if (ldx < op.n_in || ldy < op.n_out) return fail(BW_ESHAPE, "matvec_batched: leading dimension too small");
   HIPCHK(hipSetDevice(hd->device));
   hipStream_t st = static_cast<hipStream_t>(stream);
@@ -1441,6 +1447,20 @@ int bw_hip_matvec_batched(bw_hip_handle* hd, int nrhs, const double* X, int64_t 
     HIPCHK(hipGetLastError());
   }
   return BW_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bw_hip_matvec_batched(bw_hip_handle* hd, int nrhs, const double* X, int64_t ldx, double* Y,
+                          int64_t ldy, int ordering, int precision, void* stream) {
+  return batched_impl(hd, false, nrhs, X, ldx, Y, ldy, ordering, precision, stream);
+}
+
+int bw_hip_rmatvec_batched(bw_hip_handle* hd, int nrhs, const double* X, int64_t ldx, double* Y,
+                           int64_t ldy, int ordering, int precision, void* stream) {
+  return batched_impl(hd, true, nrhs, X, ldx, Y, ldy, ordering, precision, stream);
 }
 
 int bw_hip_diagonal(bw_hip_handle* hd, double* dg, int ordering) {

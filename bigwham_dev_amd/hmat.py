@@ -296,18 +296,24 @@ class HipHmat:
     def rmatvec(self, x, original: bool = True, precision: str = "fp64", out=None):
         return self.matvec(x, original, precision, out, adjoint=True)
 
-    def matvec_batched(self, X, original: bool = True, precision: str = "fp64", out=None):
-        """Y = H X for a (n, k) float64 CUDA tensor (column-major storage: X.T contiguous)."""
+    def matvec_batched(self, X, original: bool = True, precision: str = "fp64", out=None,
+                       adjoint: bool = False):
+        """Y = H X (or H^T X) for a (n, k) float64 CUDA tensor, 32 columns per
+        MFMA tile (column-major storage: X.T contiguous)."""
         import torch
         Xc = X.t().contiguous()  # k x n row-major == n x k column-major
         k = Xc.shape[0]
-        Y = torch.empty((k, self.n_rows), dtype=torch.float64, device=X.device)
+        n_in, n_out = (self.n_rows, self.n_cols) if adjoint else (self.n_cols, self.n_rows)
+        Y = torch.empty((k, n_out), dtype=torch.float64, device=X.device)
         stream = torch.cuda.current_stream(X.device).cuda_stream
-        N.check_hip(N.hip().bw_hip_matvec_batched(
-            self._h, k, C.c_void_p(Xc.data_ptr()), self.n_cols, C.c_void_p(Y.data_ptr()),
-            self.n_rows, N.BW_ORDER_ORIGINAL if original else N.BW_ORDER_PERMUTED,
-            self._prec(precision), C.c_void_p(stream)))
+        fn = N.hip().bw_hip_rmatvec_batched if adjoint else N.hip().bw_hip_matvec_batched
+        N.check_hip(fn(self._h, k, C.c_void_p(Xc.data_ptr()), n_in, C.c_void_p(Y.data_ptr()),
+                       n_out, N.BW_ORDER_ORIGINAL if original else N.BW_ORDER_PERMUTED,
+                       self._prec(precision), C.c_void_p(stream)))
         return Y.t()
+
+    def rmatvec_batched(self, X, original: bool = True, precision: str = "fp64"):
+        return self.matvec_batched(X, original, precision, adjoint=True)
 
     def diagonal(self, original: bool = True) -> np.ndarray:
         d = np.empty(max(self.n_rows, self.n_cols))

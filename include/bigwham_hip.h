@@ -96,6 +96,12 @@ int bw_hip_matvec_batched(bw_hip_handle* hd, int nrhs, const double* X, int64_t 
                           double* Y, int64_t ldy, int ordering, int precision,
                           void* stream);
 
+/* Y = H^T X for nrhs right-hand sides (requires BW_FLAG_ADJOINT); X has
+ * n_rows and Y n_cols rows.  No counterpart in the reference. */
+int bw_hip_rmatvec_batched(bw_hip_handle* hd, int nrhs, const double* X, int64_t ldx,
+                           double* Y, int64_t ldy, int ordering, int precision,
+                           void* stream);
+
 /* Diagonal (length n_rows), from the full-rank blocks that cross the
  * diagonal; ordering as above.  Host pointer. */
 int bw_hip_diagonal(bw_hip_handle* hd, double* d, int ordering);

@@ -792,7 +792,7 @@ struct bw_hip_handle {
   int cur = 0;
   double sum_ms[6] = {0, 0, 0, 0, 0, 0};
   int64_t n_timed = 0;
-  std::mutex mu;
+  std::mutex mu;  // host-side state (timing ring, lazily allocated batch buffers)
 };
 
 namespace {
@@ -971,6 +971,7 @@ int do_matvec(bw_hip_handle* hd, bool adjoint, const double* x, double* y, int o
   }
   const int64_t n_in = sd ? hd->dof * sd->n_in : op.n_in;
   const int64_t n_out = sd ? hd->dof * sd->n_out : op.n_out;
+  std::lock_guard<std::mutex> lock(hd->mu);
   hipStream_t st = static_cast<hipStream_t>(stream);
   HIPCHK(hipSetDevice(hd->device));
   const double* xd = x;
@@ -1297,6 +1298,7 @@ int bw_hip_graph_create(bw_hip_handle* hd, const double* x, double* y, int order
     return fail(BW_EINVAL, "graph_create: precision must be BW_PREC_FP64 or BW_PREC_FP32");
   DevOp& op = hd->op[adjoint ? 1 : 0][precision == BW_PREC_FP32 ? 1 : 0];
   if (!op.present) return fail(BW_ENOTSUP, "graph_create: adjoint / precision not built");
+  std::lock_guard<std::mutex> lock(hd->mu);
   HIPCHK(hipSetDevice(hd->device));
   hipStream_t cs;
   HIPCHK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
@@ -1380,6 +1382,7 @@ int batched_impl(bw_hip_handle* hd, bool adjoint, int nrhs, const double* X, int
     return fail(BW_ENOTSUP, adjoint ? "rmatvec_batched: adjoint / precision not built"
                                     : "matvec_batched: precision not built");
   if (ldx < op.n_in || ldy < op.n_out) return fail(BW_ESHAPE, "matvec_batched: leading dimension too small");
+  std::lock_guard<std::mutex> lock(hd->mu);
   HIPCHK(hipSetDevice(hd->device));
   hipStream_t st = static_cast<hipStream_t>(stream);
   const bool fp64 = precision == BW_PREC_FP64;
@@ -1487,6 +1490,7 @@ int bw_hip_sizes(bw_hip_handle* hd, int64_t* n_rows, int64_t* n_cols, int64_t* n
 
 int bw_hip_set_unroll(bw_hip_handle* hd, int unroll_fp64, int unroll_fp32) {
   if (!hd) return fail(BW_EINVAL, "set_unroll: null handle");
+  std::lock_guard<std::mutex> lock(hd->mu);
   for (int u : {unroll_fp64, unroll_fp32})
     if (u != 0 && u != 4 && u != 8 && u != 16)
       return fail(BW_EINVAL, "set_unroll: unroll must be 4, 8 or 16 (0 keeps the current one)");
@@ -1499,6 +1503,7 @@ int bw_hip_set_unroll(bw_hip_handle* hd, int unroll_fp64, int unroll_fp32) {
 
 int bw_hip_set_mfma_schedule(bw_hip_handle* hd, int schedule) {
   if (!hd) return fail(BW_EINVAL, "set_mfma_schedule: null handle");
+  std::lock_guard<std::mutex> lock(hd->mu);
   if (schedule < 0 || schedule > 3) return fail(BW_EINVAL, "set_mfma_schedule: schedule must be 0-3");
   for (int adj = 0; adj < 2; ++adj) hd->op[adj][1].mfma_schedule = schedule;
   return BW_OK;
@@ -1506,6 +1511,7 @@ int bw_hip_set_mfma_schedule(bw_hip_handle* hd, int schedule) {
 
 int bw_hip_set_timing(bw_hip_handle* hd, int enable) {
   if (!hd) return fail(BW_EINVAL, "set_timing: null handle");
+  std::lock_guard<std::mutex> lock(hd->mu);
   for (int s = 0; s < 2; ++s)
     if (hd->pending[s]) HIPCHK(hipEventSynchronize(hd->evs[s][5]));
   hd->pending[0] = hd->pending[1] = false;
@@ -1518,6 +1524,7 @@ int bw_hip_set_timing(bw_hip_handle* hd, int enable) {
 
 int bw_hip_timing_stats(bw_hip_handle* hd, double* avg_ms6, int64_t* count) {
   if (!hd || !avg_ms6) return fail(BW_EINVAL, "timing_stats: null argument");
+  std::lock_guard<std::mutex> lock(hd->mu);
   for (int s = 0; s < 2; ++s) {
     const int set = (hd->cur + s) & 1;  // oldest first
     if (hd->pending[set]) {

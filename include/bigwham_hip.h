@@ -15,8 +15,10 @@
  * All vectors are caller-owned.  Calls are asynchronous on the given HIP
  * stream when the vectors are device pointers; with host pointers the call
  * copies in, computes and copies out before returning (like
- * HmatCuda::matvec(il::ArrayView), hmat_cuda.cc:2349-2394).  Calls on one
- * handle share its scratch vectors and must not overlap.
+ * HmatCuda::matvec(il::ArrayView), hmat_cuda.cc:2349-2394).  Host threads
+ * may call one handle concurrently (its host state is locked), but the GPU
+ * work of all calls on a handle shares its scratch vectors: calls issued on
+ * different streams must be ordered by the caller.
  * Every function returns BW_OK (0) or a negative BW_E* code and never exits
  * the process; bw_hip_last_error() gives the message of the calling thread's
  * last failure.

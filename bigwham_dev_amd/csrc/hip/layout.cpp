@@ -325,12 +325,18 @@ void build_layout(const OpView& v, const LayoutOptions& opt, Layout* L) {
              v.lr_c1[order[j]] == v.lr_c1[order[i]])
         ++j;
       const int64_t c0 = v.lr_c0[order[i]], ncols = d * (v.lr_c1[order[i]] - c0);
-      // chunks of at most maxr rows, packed first-fit decreasing into slabs
+      // each factor's rows cut into ceil(r / maxr) chunks of even height (a
+      // rank of 90 gives 45 + 45, not 64 + 26), packed first-fit decreasing
       std::vector<StackPart> chunks;
-      for (size_t q = i; q < j; ++q)
-        for (int64_t r0 = 0; r0 < v.lr_rank[order[q]]; r0 += maxr)
-          chunks.push_back({order[q], static_cast<int32_t>(r0),
-                            static_cast<int32_t>(std::min(maxr, v.lr_rank[order[q]] - r0))});
+      for (size_t q = i; q < j; ++q) {
+        const int64_t r = v.lr_rank[order[q]];
+        const int64_t nsl = (r + maxr - 1) / maxr;
+        for (int64_t c = 0, r0 = 0; c < nsl; ++c) {
+          const int64_t n = (r - r0) / (nsl - c);
+          chunks.push_back({order[q], static_cast<int32_t>(r0), static_cast<int32_t>(n)});
+          r0 += n;
+        }
+      }
       std::stable_sort(chunks.begin(), chunks.end(),
                        [](const StackPart& a, const StackPart& b) { return a.nrows > b.nrows; });
       std::vector<std::vector<StackPart>> bins;

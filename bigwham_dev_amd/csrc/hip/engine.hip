@@ -1,9 +1,10 @@
 // MI355X (gfx950) hierarchical-matrix matvec engine and its C-ABI.
 //
-// One matvec y = H x is four kernels on the caller's stream:
+// One matvec y = H x is five kernels on the caller's stream:
 //   k_gather   x -> xt[0:n]   (permutation of matvecOriginal, hmat.cc:586-593,
 //                              or a copy; converts to the factor precision)
-//   k_slab     pass 1: t = W x   (W_k = B_k), written into xt[xt_t : ...]
+//   k_slab     pass 1: t = W x   (W_k = B_k), each row written through the
+//              row map into xt[xt_t : ...]
 //   k_reduce   sum of split pass-1 work items (only if any)
 //   k_slab     pass 2: y = F x + U t, each y row written once, with the
 //              permutation of matvecOriginal (hmat.cc:618-626) fused in

@@ -311,9 +311,6 @@ __global__ __launch_bounds__(256) void k_gather_b(const double* __restrict__ X, 
 }
 
 // KU group pairs per step; PIPE: the loads of step i+1 are issued before the
-// MFMAs of step i.  Addresses are clamped into the work item and invalid
-// lanes zeroed by selects, so the loop has no divergent branches.
-// KU group pairs per step; PIPE: the loads of step i+1 are issued before the
 // MFMAs of step i.  Addresses are clamped into the work item, so every lane
 // loads valid data; a group past the end gets B = 0 (its A rows then add
 // nothing), and rows past R are never stored.  The mask is applied at the

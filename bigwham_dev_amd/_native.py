@@ -100,6 +100,9 @@ def hip() -> C.CDLL:
         lib.bw_hip_matvec_batched.argtypes = [_vp, C.c_int, _vp, C.c_int64, _vp, C.c_int64,
                                               C.c_int, C.c_int, _vp]
         lib.bw_hip_rmatvec_batched.argtypes = lib.bw_hip_matvec_batched.argtypes
+        lib.bw_hip_matvec_batched_strided.argtypes = [_vp, C.c_int, C.c_int, _vp, C.c_int64,
+                                                      C.c_int64, _vp, C.c_int64, C.c_int64,
+                                                      C.c_int, C.c_int, _vp]
         lib.bw_hip_diagonal.argtypes = [_vp, _d, C.c_int]
         lib.bw_hip_sizes.argtypes = [_vp, _i64, _i64, _i64, _i64, _i64, _i64]
         lib.bw_hip_set_tuning.argtypes = [C.c_int64, C.c_int64]
@@ -170,7 +173,7 @@ HOST_SYMBOLS = ["bwh_create", "bwh_create_gaussian", "bwh_build_pattern", "bwh_b
                 "bwh_rotation_matrices", "bwh_dense_block", "bwh_max_aca_error",
                 "bwh_timings", "bwh_free", "bwh_last_error", "bwh_influence"]
 HIP_SYMBOLS = ["bw_hip_create", "bw_hip_matvec", "bw_hip_rmatvec", "bw_hip_matvec_batched",
-               "bw_hip_rmatvec_batched",
+               "bw_hip_rmatvec_batched", "bw_hip_matvec_batched_strided",
                "bw_hip_diagonal", "bw_hip_sizes", "bw_hip_set_tuning", "bw_hip_set_stacking", "bw_hip_set_unroll", "bw_hip_set_mfma_schedule",
                "bw_hip_set_timing",
                "bw_hip_timing_stats", "bw_hip_destroy", "bw_hip_last_error",

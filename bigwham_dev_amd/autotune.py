@@ -25,7 +25,7 @@ import os
 from dataclasses import dataclass, field
 
 UNROLLS = (4, 8, 16)
-MFMA_SCHEDULES = (0, 1, 2, 3)
+MFMA_SCHEDULES = (0, 1, 3, 4, 5, 6, 7)
 WI_BYTES = (64 << 10, 96 << 10, 192 << 10, 384 << 10)
 
 

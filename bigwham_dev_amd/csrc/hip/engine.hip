@@ -296,7 +296,8 @@ constexpr int kRhs = 32;
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_gather_b(const double* __restrict__ X, int64_t ldx,
+__global__ __launch_bounds__(256) void k_gather_b(const double* __restrict__ X, int64_t xr,
+                                                  int64_t xc,
                                                   int nrhs, T* __restrict__ XB, int64_t n,
                                                   const int64_t* __restrict__ perm, int dof) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * (blockDim.x / kRhs) + threadIdx.x / kRhs;
@@ -307,7 +308,7 @@ __global__ __launch_bounds__(256) void k_gather_b(const double* __restrict__ X, 
     const int64_t p = i / dof;
     src = dof * perm[p] + (i - p * dof);
   }
-  XB[i * kRhs + c] = c < nrhs ? static_cast<T>(X[src + c * ldx]) : T(0);
+  XB[i * kRhs + c] = c < nrhs ? static_cast<T>(X[src * xr + c * xc]) : T(0);
 }
 
 // KU group pairs per step; PIPE: the loads of step i+1 are issued before the
@@ -386,12 +387,80 @@ __device__ __forceinline__ void mfma_stream(const flt4* __restrict__ base,
   }
 }
 
-template <int PASS, int KU, bool PIPE>
+// Deep variant: D step buffers in registers used as a ring.  At stage j the
+// loads of step s + D - 1 go out before the MFMAs of step s, so D - 1 steps
+// of factor and input loads are in flight per wave while its MFMAs run (the
+// two-buffer pipeline above keeps one).  Steps past the end load clamped,
+// in-bounds addresses and skip their MFMAs (a wave-uniform branch).  zz[k]
+// holds the input offsets of the next step to be loaded into buffer k.
+template <int KU, int D, bool TWO>
+__device__ __forceinline__ void mfma_stream_deep(const flt4* __restrict__ base,
+                                                 const int32_t* __restrict__ gi,
+                                                 const float* __restrict__ XB, int ng, int R,
+                                                 int half, int r0, floatx16& acc0,
+                                                 floatx16& acc1) {
+  constexpr int G = 2 * KU;  // groups per step
+  const int rA0 = min(r0, R - 1), rA1 = min(r0 + 32, R - 1);
+  const int nsteps = (ng + G - 1) / G;
+  flt4 a0[D][KU], a1[D][KU];
+  float b[D][KU][4];
+  int zz[D][KU];
+  auto gz = [&](int step, int u) { return gi[min(step * G + 2 * u + half, ng - 1)]; };
+  auto load = [&](int step, const int (&z)[KU], flt4 (&x0)[KU], flt4 (&x1)[KU],
+                  float (&bb)[KU][4]) {
+#pragma unroll
+    for (int u = 0; u < KU; ++u) {
+      const int ga = min(step * G + 2 * u + half, ng - 1);
+      x0[u] = __builtin_nontemporal_load(base + ga * R + rA0);
+      if (TWO) x1[u] = __builtin_nontemporal_load(base + ga * R + rA1);
+#pragma unroll
+      for (int s = 0; s < 4; ++s) bb[u][s] = XB[static_cast<int64_t>(z[u] + s) * kRhs + r0];
+    }
+  };
+  auto compute = [&](int step, const flt4 (&x0)[KU], const flt4 (&x1)[KU],
+                     const float (&bb)[KU][4]) {
+#pragma unroll
+    for (int u = 0; u < KU; ++u) {
+      const bool ok = step * G + 2 * u + half < ng;
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const float bv = ok ? bb[u][s] : 0.0f;
+        acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(x0[u][s], bv, acc0, 0, 0, 0);
+        if (TWO) acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(x1[u][s], bv, acc1, 0, 0, 0);
+      }
+    }
+  };
+#pragma unroll
+  for (int k = 0; k < D; ++k)
+#pragma unroll
+    for (int u = 0; u < KU; ++u) zz[k][u] = gz(k, u);
+#pragma unroll
+  for (int k = 0; k < D - 1; ++k) {
+    load(k, zz[k], a0[k], a1[k], b[k]);
+#pragma unroll
+    for (int u = 0; u < KU; ++u) zz[k][u] = gz(k + D, u);
+  }
+  for (int s0 = 0; s0 < nsteps; s0 += D) {
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      const int s = s0 + j;
+      const int k = (j + D - 1) % D;  // buffer freed by the previous stage
+      load(s + D - 1, zz[k], a0[k], a1[k], b[k]);
+#pragma unroll
+      for (int u = 0; u < KU; ++u) zz[k][u] = gz(s + 2 * D - 1, u);
+      __builtin_amdgcn_sched_barrier(0);
+      if (s < nsteps) compute(s, a0[j], a1[j], b[j]);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+}
+
+template <int PASS, int KU, bool PIPE, int D = 0>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_slab_mfma(
     const bwl::WorkItem* __restrict__ wis, XcdSplit xs, const int32_t* __restrict__ gidx,
     const float* __restrict__ data, const float* __restrict__ XB,
     float* __restrict__ tout,              // pass 1
-    double* __restrict__ Y, int64_t ldy,  // pass 2
+    double* __restrict__ Y, int64_t yr, int64_t yc,  // pass 2 (row, column strides)
     int nrhs, float* __restrict__ part, const int64_t* __restrict__ perm, int dof,
     const int32_t* __restrict__ rowmap) {
   const int lane = threadIdx.x & 63;
@@ -407,10 +476,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   floatx16 acc0 = {0}, acc1 = {0};
   const bool two = R > 32;
   if (ng > 0) {
-    if (two)
-      mfma_stream<KU, PIPE, true>(base, gi, XB, ng, R, half, r0, acc0, acc1);
-    else
-      mfma_stream<KU, PIPE, false>(base, gi, XB, ng, R, half, r0, acc0, acc1);
+    if constexpr (D > 0) {
+      if (two)
+        mfma_stream_deep<KU, D, true>(base, gi, XB, ng, R, half, r0, acc0, acc1);
+      else
+        mfma_stream_deep<KU, D, false>(base, gi, XB, ng, R, half, r0, acc0, acc1);
+    } else {
+      if (two)
+        mfma_stream<KU, PIPE, true>(base, gi, XB, ng, R, half, r0, acc0, acc1);
+      else
+        mfma_stream<KU, PIPE, false>(base, gi, XB, ng, R, half, r0, acc0, acc1);
+    }
   }
   const int n = lane & 31;
   auto row_of = [&](int j) { return 8 * (j >> 2) + 4 * half + (j & 3); };
@@ -419,7 +495,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     if (PASS == 1) {
       tout[static_cast<int64_t>(rowmap[out_row + row]) * kRhs + n] = v;
     } else {
-      Y[out_index<double>(out_row + row, perm, dof) + n * ldy] = static_cast<double>(v);
+      Y[out_index<double>(out_row + row, perm, dof) * yr + n * yc] = static_cast<double>(v);
     }
   };
   if (wi.part_off < 0) {
@@ -443,31 +519,47 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 template <int PASS>
 void launch_mfma(int schedule, int blocks, hipStream_t st, const bwl::WorkItem* wis, XcdSplit xs,
                  const int32_t* gidx, const float* data, const float* XB, float* tout, double* Y,
-                 int64_t ldy, int nrhs, float* part, const int64_t* perm, int dof,
+                 int64_t yr, int64_t yc, int nrhs, float* part, const int64_t* perm, int dof,
                  const int32_t* rowmap) {
   switch (schedule) {
     case 1:
       hipLaunchKernelGGL((k_slab_mfma<PASS, 2, true>), dim3(blocks), dim3(256), 0, st, wis, xs,
-                         gidx, data, XB, tout, Y, ldy, nrhs, part, perm, dof, rowmap);
+                         gidx, data, XB, tout, Y, yr, yc, nrhs, part, perm, dof, rowmap);
       break;
     case 2:
       hipLaunchKernelGGL((k_slab_mfma<PASS, 4, true>), dim3(blocks), dim3(256), 0, st, wis, xs,
-                         gidx, data, XB, tout, Y, ldy, nrhs, part, perm, dof, rowmap);
+                         gidx, data, XB, tout, Y, yr, yc, nrhs, part, perm, dof, rowmap);
       break;
     case 3:
       hipLaunchKernelGGL((k_slab_mfma<PASS, 8, false>), dim3(blocks), dim3(256), 0, st, wis, xs,
-                         gidx, data, XB, tout, Y, ldy, nrhs, part, perm, dof, rowmap);
+                         gidx, data, XB, tout, Y, yr, yc, nrhs, part, perm, dof, rowmap);
+      break;
+    case 4:
+      hipLaunchKernelGGL((k_slab_mfma<PASS, 2, true, 3>), dim3(blocks), dim3(256), 0, st, wis, xs,
+                         gidx, data, XB, tout, Y, yr, yc, nrhs, part, perm, dof, rowmap);
+      break;
+    case 5:
+      hipLaunchKernelGGL((k_slab_mfma<PASS, 2, true, 4>), dim3(blocks), dim3(256), 0, st, wis, xs,
+                         gidx, data, XB, tout, Y, yr, yc, nrhs, part, perm, dof, rowmap);
+      break;
+    case 6:
+      hipLaunchKernelGGL((k_slab_mfma<PASS, 1, true, 6>), dim3(blocks), dim3(256), 0, st, wis, xs,
+                         gidx, data, XB, tout, Y, yr, yc, nrhs, part, perm, dof, rowmap);
+      break;
+    case 7:
+      hipLaunchKernelGGL((k_slab_mfma<PASS, 1, true, 8>), dim3(blocks), dim3(256), 0, st, wis, xs,
+                         gidx, data, XB, tout, Y, yr, yc, nrhs, part, perm, dof, rowmap);
       break;
     default:
       hipLaunchKernelGGL((k_slab_mfma<PASS, 4, false>), dim3(blocks), dim3(256), 0, st, wis, xs,
-                         gidx, data, XB, tout, Y, ldy, nrhs, part, perm, dof, rowmap);
+                         gidx, data, XB, tout, Y, yr, yc, nrhs, part, perm, dof, rowmap);
   }
 }
 
 template <int PASS, typename T>
 __global__ __launch_bounds__(256) void k_reduce_b(const bwl::Reduce* __restrict__ reds, int nred,
                                                   const T* __restrict__ part, T* tout,
-                                                  double* Y, int64_t ldy, int nrhs,
+                                                  double* Y, int64_t yr, int64_t yc, int nrhs,
                                                   const int64_t* __restrict__ perm, int dof,
                                                   const int32_t* __restrict__ rowmap) {
   // one workgroup per split slab; a thread sums one 16-byte vector of
@@ -505,7 +597,7 @@ __global__ __launch_bounds__(256) void k_reduce_b(const bwl::Reduce* __restrict_
       const int64_t o = out_index<double>(static_cast<int64_t>(rd.out_row) + row, perm, dof);
 #pragma unroll
       for (int k = 0; k < VW; ++k)
-        if (n0 + k < nrhs) Y[o + (n0 + k) * ldy] = static_cast<double>(sum[k]);
+        if (n0 + k < nrhs) Y[o * yr + (n0 + k) * yc] = static_cast<double>(sum[k]);
     }
   }
 }
@@ -526,7 +618,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     const bwl::WorkItem* __restrict__ wis, XcdSplit xs, const int32_t* __restrict__ gidx,
     const double* __restrict__ data, const double* __restrict__ XB,
     double* __restrict__ tout,             // pass 1
-    double* __restrict__ Y, int64_t ldy,  // pass 2
+    double* __restrict__ Y, int64_t yr, int64_t yc,  // pass 2 (row, column strides)
     int nrhs, double* __restrict__ part, const int64_t* __restrict__ perm, int dof,
     const int32_t* __restrict__ rowmap) {
   const int lane = threadIdx.x & 63;
@@ -589,7 +681,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
           if (PASS == 1)
             tout[static_cast<int64_t>(rowmap[wi.out_row + row]) * kRhs + n] = val;
           else
-            Y[out_index<double>(static_cast<int64_t>(wi.out_row) + row, perm, dof) + n * ldy] = val;
+            Y[out_index<double>(static_cast<int64_t>(wi.out_row) + row, perm, dof) * yr + n * yc] = val;
         }
       }
   }
@@ -622,7 +714,7 @@ struct DevOp {
   int64_t stream_bytes = 0;
   int64_t part_elems[2] = {0, 0};
   int unroll = 16;            // loads in flight per lane in k_slab (4, 8 or 16)
-  int mfma_schedule = 0;      // k_slab_mfma variant (0-3)
+  int mfma_schedule = 0;      // k_slab_mfma variant (0-7)
   // many right-hand sides (FP32 MFMA path), allocated on first use
   void* xb = nullptr;        // xt_size x kRhs
   void* partb[2] = {nullptr, nullptr};
@@ -1367,9 +1459,14 @@ int bw_hip_rmatvec(bw_hip_handle* hd, const double* x, double* y, int ordering, 
 
 namespace {
 
-int batched_impl(bw_hip_handle* hd, bool adjoint, int nrhs, const double* X, int64_t ldx,
-                 double* Y, int64_t ldy, int ordering, int precision, void* stream) {
+// X(i, j) = X[i * xr + j * xc] (input index i, right-hand side j); Y likewise
+// with (yr, yc).  Column-major is (1, ld), row-major (nrhs, 1).
+int batched_impl(bw_hip_handle* hd, bool adjoint, int nrhs, const double* X, int64_t xr,
+                 int64_t xc, double* Y, int64_t yr, int64_t yc, int ordering, int precision,
+                 void* stream, bool leading_dims = false) {
   if (!hd || !X || !Y || nrhs <= 0) return fail(BW_EINVAL, "matvec_batched: bad argument");
+  if (xr < 1 || xc < 1 || yr < 1 || yc < 1)
+    return fail(BW_ESHAPE, "matvec_batched: strides must be positive");
   if (hd->sel.on) return fail(BW_ENOTSUP, "matvec_batched: not available for a selection");
   if (ordering != BW_ORDER_PERMUTED && ordering != BW_ORDER_ORIGINAL)
     return fail(BW_EINVAL, "matvec_batched: bad ordering");
@@ -1379,7 +1476,12 @@ int batched_impl(bw_hip_handle* hd, bool adjoint, int nrhs, const double* X, int
   if (!op.present)
     return fail(BW_ENOTSUP, adjoint ? "rmatvec_batched: adjoint / precision not built"
                                     : "matvec_batched: precision not built");
-  if (ldx < op.n_in || ldy < op.n_out) return fail(BW_ESHAPE, "matvec_batched: leading dimension too small");
+  if (leading_dims && (xc < op.n_in || yc < op.n_out))
+    return fail(BW_ESHAPE, "matvec_batched: leading dimension too small");
+  // distinct outputs must not share an address: columns apart by at least a
+  // column's extent, or rows apart by at least a row's
+  if (!(yc >= op.n_out * yr || yr >= static_cast<int64_t>(nrhs) * yc))
+    return fail(BW_ESHAPE, "matvec_batched: output strides overlap");
   std::lock_guard<std::mutex> lock(hd->mu);
   HIPCHK(hipSetDevice(hd->device));
   hipStream_t st = static_cast<hipStream_t>(stream);
@@ -1398,8 +1500,8 @@ int batched_impl(bw_hip_handle* hd, bool adjoint, int nrhs, const double* X, int
   const int dof = hd->dof;
   for (int k0 = 0; k0 < nrhs; k0 += kRhs) {
     const int nk = std::min(kRhs, nrhs - k0);
-    const double* Xk = X + static_cast<int64_t>(k0) * ldx;
-    double* Yk = Y + static_cast<int64_t>(k0) * ldy;
+    const double* Xk = X + static_cast<int64_t>(k0) * xc;
+    double* Yk = Y + static_cast<int64_t>(k0) * yc;
     const int64_t rows_per_block = 256 / kRhs;
     const int gblocks = static_cast<int>((op.n_in + rows_per_block - 1) / rows_per_block);
     const int64_t* pin = orig ? op.perm_in : nullptr;
@@ -1408,42 +1510,42 @@ int batched_impl(bw_hip_handle* hd, bool adjoint, int nrhs, const double* X, int
       double* xb = static_cast<double*>(op.xb);
       double* pb0 = static_cast<double*>(op.partb[0]);
       double* pb1 = static_cast<double*>(op.partb[1]);
-      hipLaunchKernelGGL(k_gather_b<double>, dim3(gblocks), dim3(256), 0, st, Xk, ldx, nk, xb,
+      hipLaunchKernelGGL(k_gather_b<double>, dim3(gblocks), dim3(256), 0, st, Xk, xr, xc, nk, xb,
                          op.n_in, pin, dof);
       if (op.p1.nwi > 0)
         hipLaunchKernelGGL(k_slab_mfma64<1>, dim3(op.p1.blocks), dim3(256), 0, st, op.p1.wis,
                            op.p1.xs, op.p1.gidx, static_cast<const double*>(op.p1.data), xb, xb,
-                           nullptr, 0, kRhs, pb0, nullptr, dof, op.p1.rowmap);
+                           nullptr, 0, 0, kRhs, pb0, nullptr, dof, op.p1.rowmap);
       if (op.p1.nred > 0)
         hipLaunchKernelGGL((k_reduce_b<1, double>), dim3(op.p1.nred), dim3(256), 0, st, op.p1.reds,
-                           op.p1.nred, pb0, xb, nullptr, 0, kRhs, nullptr, dof, op.p1.rowmap);
+                           op.p1.nred, pb0, xb, nullptr, 0, 0, kRhs, nullptr, dof, op.p1.rowmap);
       if (op.p2.nwi > 0)
         hipLaunchKernelGGL(k_slab_mfma64<2>, dim3(op.p2.blocks), dim3(256), 0, st, op.p2.wis,
                            op.p2.xs, op.p2.gidx, static_cast<const double*>(op.p2.data), xb,
-                           nullptr, Yk, ldy, nk, pb1, pout, dof, nullptr);
+                           nullptr, Yk, yr, yc, nk, pb1, pout, dof, nullptr);
       if (op.p2.nred > 0)
         hipLaunchKernelGGL((k_reduce_b<2, double>), dim3(op.p2.nred), dim3(256), 0, st, op.p2.reds,
-                           op.p2.nred, pb1, nullptr, Yk, ldy, nk, pout, dof, nullptr);
+                           op.p2.nred, pb1, nullptr, Yk, yr, yc, nk, pout, dof, nullptr);
     } else {
       float* xb = static_cast<float*>(op.xb);
       float* pb0 = static_cast<float*>(op.partb[0]);
       float* pb1 = static_cast<float*>(op.partb[1]);
-      hipLaunchKernelGGL(k_gather_b<float>, dim3(gblocks), dim3(256), 0, st, Xk, ldx, nk, xb,
+      hipLaunchKernelGGL(k_gather_b<float>, dim3(gblocks), dim3(256), 0, st, Xk, xr, xc, nk, xb,
                          op.n_in, pin, dof);
       if (op.p1.nwi > 0)
         launch_mfma<1>(op.mfma_schedule, op.p1.blocks, st, op.p1.wis, op.p1.xs, op.p1.gidx,
-                       static_cast<const float*>(op.p1.data), xb, xb, nullptr, 0, kRhs, pb0,
+                       static_cast<const float*>(op.p1.data), xb, xb, nullptr, 0, 0, kRhs, pb0,
                        nullptr, dof, op.p1.rowmap);
       if (op.p1.nred > 0)
         hipLaunchKernelGGL((k_reduce_b<1, float>), dim3(op.p1.nred), dim3(256), 0, st, op.p1.reds,
-                           op.p1.nred, pb0, xb, nullptr, 0, kRhs, nullptr, dof, op.p1.rowmap);
+                           op.p1.nred, pb0, xb, nullptr, 0, 0, kRhs, nullptr, dof, op.p1.rowmap);
       if (op.p2.nwi > 0)
         launch_mfma<2>(op.mfma_schedule, op.p2.blocks, st, op.p2.wis, op.p2.xs, op.p2.gidx,
-                       static_cast<const float*>(op.p2.data), xb, nullptr, Yk, ldy, nk, pb1, pout,
+                       static_cast<const float*>(op.p2.data), xb, nullptr, Yk, yr, yc, nk, pb1, pout,
                        dof, nullptr);
       if (op.p2.nred > 0)
         hipLaunchKernelGGL((k_reduce_b<2, float>), dim3(op.p2.nred), dim3(256), 0, st, op.p2.reds,
-                           op.p2.nred, pb1, nullptr, Yk, ldy, nk, pout, dof, nullptr);
+                           op.p2.nred, pb1, nullptr, Yk, yr, yc, nk, pout, dof, nullptr);
     }
     HIPCHK(hipGetLastError());
   }
@@ -1456,12 +1558,20 @@ extern "C" {
 
 int bw_hip_matvec_batched(bw_hip_handle* hd, int nrhs, const double* X, int64_t ldx, double* Y,
                           int64_t ldy, int ordering, int precision, void* stream) {
-  return batched_impl(hd, false, nrhs, X, ldx, Y, ldy, ordering, precision, stream);
+  return batched_impl(hd, false, nrhs, X, 1, ldx, Y, 1, ldy, ordering, precision, stream, true);
 }
 
 int bw_hip_rmatvec_batched(bw_hip_handle* hd, int nrhs, const double* X, int64_t ldx, double* Y,
                            int64_t ldy, int ordering, int precision, void* stream) {
-  return batched_impl(hd, true, nrhs, X, ldx, Y, ldy, ordering, precision, stream);
+  return batched_impl(hd, true, nrhs, X, 1, ldx, Y, 1, ldy, ordering, precision, stream, true);
+}
+
+int bw_hip_matvec_batched_strided(bw_hip_handle* hd, int adjoint, int nrhs, const double* X,
+                                  int64_t x_row_stride, int64_t x_col_stride, double* Y,
+                                  int64_t y_row_stride, int64_t y_col_stride, int ordering,
+                                  int precision, void* stream) {
+  return batched_impl(hd, adjoint != 0, nrhs, X, x_row_stride, x_col_stride, Y, y_row_stride,
+                      y_col_stride, ordering, precision, stream);
 }
 
 int bw_hip_diagonal(bw_hip_handle* hd, double* dg, int ordering) {
@@ -1502,7 +1612,7 @@ int bw_hip_set_unroll(bw_hip_handle* hd, int unroll_fp64, int unroll_fp32) {
 int bw_hip_set_mfma_schedule(bw_hip_handle* hd, int schedule) {
   if (!hd) return fail(BW_EINVAL, "set_mfma_schedule: null handle");
   std::lock_guard<std::mutex> lock(hd->mu);
-  if (schedule < 0 || schedule > 3) return fail(BW_EINVAL, "set_mfma_schedule: schedule must be 0-3");
+  if (schedule < 0 || schedule > 7) return fail(BW_EINVAL, "set_mfma_schedule: schedule must be 0-7");
   for (int adj = 0; adj < 2; ++adj) hd->op[adj][1].mfma_schedule = schedule;
   return BW_OK;
 }

@@ -299,18 +299,28 @@ class HipHmat:
     def matvec_batched(self, X, original: bool = True, precision: str = "fp64", out=None,
                        adjoint: bool = False):
         """Y = H X (or H^T X) for a (n, k) float64 CUDA tensor, 32 columns per
-        MFMA tile (column-major storage: X.T contiguous)."""
+        MFMA tile.  X may have any positive strides (row-major, column-major,
+        a column slice) and is read in place; Y is row-major like torch's
+        default unless ``out`` (an (n_out, k) float64 tensor) is given."""
         import torch
-        Xc = X.t().contiguous()  # k x n row-major == n x k column-major
-        k = Xc.shape[0]
         n_in, n_out = (self.n_rows, self.n_cols) if adjoint else (self.n_cols, self.n_rows)
-        Y = torch.empty((k, n_out), dtype=torch.float64, device=X.device)
+        if X.dim() != 2 or X.shape[0] != n_in or X.dtype != torch.float64 or not X.is_cuda:
+            raise ValueError(f"matvec_batched: X must be a ({n_in}, k) float64 CUDA tensor")
+        k = X.shape[1]
+        if min(X.stride()) < 1:  # broadcast (stride 0) views have no distinct entries
+            X = X.contiguous()
+        Y = out if out is not None else torch.empty((n_out, k), dtype=torch.float64,
+                                                     device=X.device)
+        if Y.shape != (n_out, k) or Y.dtype != torch.float64 or Y.device != X.device:
+            raise ValueError(f"matvec_batched: out must be a ({n_out}, {k}) float64 tensor "
+                             "on the device of X")
         stream = torch.cuda.current_stream(X.device).cuda_stream
-        fn = N.hip().bw_hip_rmatvec_batched if adjoint else N.hip().bw_hip_matvec_batched
-        N.check_hip(fn(self._h, k, C.c_void_p(Xc.data_ptr()), n_in, C.c_void_p(Y.data_ptr()),
-                       n_out, N.BW_ORDER_ORIGINAL if original else N.BW_ORDER_PERMUTED,
-                       self._prec(precision), C.c_void_p(stream)))
-        return Y.t()
+        N.check_hip(N.hip().bw_hip_matvec_batched_strided(
+            self._h, int(adjoint), k, C.c_void_p(X.data_ptr()), X.stride(0), X.stride(1),
+            C.c_void_p(Y.data_ptr()), Y.stride(0), Y.stride(1),
+            N.BW_ORDER_ORIGINAL if original else N.BW_ORDER_PERMUTED,
+            self._prec(precision), C.c_void_p(stream)))
+        return Y
 
     def rmatvec_batched(self, X, original: bool = True, precision: str = "fp64"):
         return self.matvec_batched(X, original, precision, adjoint=True)

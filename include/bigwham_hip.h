@@ -104,6 +104,17 @@ int bw_hip_rmatvec_batched(bw_hip_handle* hd, int nrhs, const double* X, int64_t
                            double* Y, int64_t ldy, int ordering, int precision,
                            void* stream);
 
+/* Both of the above with any element strides: X(i, j) = X[i * x_row_stride +
+ * j * x_col_stride] (i the vector index, j the right-hand side), Y likewise,
+ * so row-major (nrhs, 1) and column-major (1, ld) operands run without a
+ * transpose.  adjoint != 0 applies H^T.  Strides must be positive and the
+ * output ones must not make two entries of Y share an address (BW_ESHAPE).
+ * No counterpart in the reference (it has no multi-RHS product). */
+int bw_hip_matvec_batched_strided(bw_hip_handle* hd, int adjoint, int nrhs, const double* X,
+                                  int64_t x_row_stride, int64_t x_col_stride, double* Y,
+                                  int64_t y_row_stride, int64_t y_col_stride, int ordering,
+                                  int precision, void* stream);
+
 /* Diagonal (length n_rows), from the full-rank blocks that cross the
  * diagonal; ordering as above.  Host pointer. */
 int bw_hip_diagonal(bw_hip_handle* hd, double* d, int ordering);
@@ -164,7 +175,9 @@ int bw_hip_set_unroll(bw_hip_handle* hd, int unroll_fp64, int unroll_fp32);
 /* Schedule of the FP32 MFMA kernel of bw_hip_matvec_batched: group pairs per
  * step and whether the next step's loads are issued before the current
  * step's MFMAs (0: 4 pairs; 1: 2 pairs, pipelined; 2: 4 pairs, pipelined;
- * 3: 8 pairs).  Chosen by bigwham_dev_amd.autotune. */
+ * 3: 8 pairs; 4-7: a ring of D step buffers with D - 1 steps of loads in
+ * flight: 4: 2 pairs, D = 3; 5: 2 pairs, D = 4; 6: 1 pair, D = 6; 7: 1 pair,
+ * D = 8).  Chosen by bigwham_dev_amd.autotune. */
 int bw_hip_set_mfma_schedule(bw_hip_handle* hd, int schedule);
 
 /* Per-kernel timing.  With timing enabled every matvec records HIP events

@@ -155,16 +155,74 @@ def test_autotune_unroll(gpu, cases, tmp_path, monkeypatch):
     assert r2.unroll == r.unroll and not r2.ms
 
 
-@pytest.mark.parametrize("schedule", [0, 1, 2, 3])
-def test_batched_mfma_schedules(gpu, cases, schedule):
-    """Every schedule the autotuner may pick gives the same FP32 results."""
+@pytest.mark.parametrize("name", ["3DR0", "gaussian"])
+def test_batched_mfma_schedules(gpu, cases, name):
+    """Every schedule the autotuner may pick keeps FP32 parity; they issue
+    the same MFMAs in the same order per accumulator, so their results agree
+    bit for bit.  3DR0 has 64-row slabs (two 32-row tiles), the Gaussian
+    matrix (dof 1, leaf 32) slabs of at most 32 rows; the small work items
+    leave fewer steps than the deep schedules' ring holds."""
     import torch
-    h = cases["3DR0"]
-    print(f"mfma schedule {schedule}", flush=True)
+    h = cases[name]
     dev = HipHmat(h, precisions=("fp32",))
-    dev.set_mfma_schedule(schedule)
     X = torch.from_numpy(np.random.default_rng(13).uniform(-1, 1, (h.size, 32))).to(gpu)
-    Y = dev.matvec_batched(X, precision="fp32").cpu().numpy()
-    for j in (0, 17, 31):
-        ref = O.matvec_original(h.flat(), X[:, j].cpu().numpy())
-        assert rel(Y[:, j], ref) <= TOL32
+    first = None
+    for schedule in range(8):
+        print(f"case {name} mfma schedule {schedule}", flush=True)
+        dev.set_mfma_schedule(schedule)
+        Y = dev.matvec_batched(X, precision="fp32").cpu().numpy()
+        for j in (0, 17, 31):
+            ref = O.matvec_original(h.flat(), X[:, j].cpu().numpy())
+            assert rel(Y[:, j], ref) <= TOL32, (schedule, j)
+        if first is None:
+            first = Y
+        else:
+            assert np.array_equal(Y, first), schedule
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_batched_strides(gpu, cases, precision):
+    """Row-major, column-major and sliced operands give the same bits as
+    the column-major leading-dimension entry point, and bad strides fail."""
+    import ctypes as C
+    import torch
+    from bigwham_dev_amd import _native as N
+    h = cases["3DR0"]
+    n = h.size
+    print(f"strides {precision} n={n}", flush=True)
+    dev = HipHmat(h, precisions=("fp64", "fp32"), adjoint=True)
+    prec = N.BW_PREC_FP64 if precision == "fp64" else N.BW_PREC_FP32
+    Xh = np.random.default_rng(21).uniform(-1, 1, (n, 20))
+    Xr = torch.from_numpy(Xh).to(gpu)                     # row-major (20, 1)
+    Xc = Xr.t().contiguous().t()                          # column-major (1, n)
+    assert Xc.stride() == (1, n)
+    # the leading-dimension entry point, column-major in and out
+    Yld = torch.empty((20, n), dtype=torch.float64, device=gpu)
+    stream = torch.cuda.current_stream(gpu).cuda_stream
+    N.check_hip(N.hip().bw_hip_matvec_batched(
+        dev._h, 20, C.c_void_p(Xc.data_ptr()), n, C.c_void_p(Yld.data_ptr()), n,
+        N.BW_ORDER_ORIGINAL, prec, C.c_void_p(stream)))
+    ref = Yld.t().cpu().numpy()
+    Yr = dev.matvec_batched(Xr, precision=precision)
+    assert Yr.stride() == (20, 1)
+    assert np.array_equal(Yr.cpu().numpy(), ref)
+    assert np.array_equal(dev.matvec_batched(Xc, precision=precision).cpu().numpy(), ref)
+    # a column slice of a wider row-major matrix, into a column-major output
+    Xw = torch.from_numpy(np.random.default_rng(22).uniform(-1, 1, (n, 30))).to(gpu)
+    Xw[:, 5:25] = Xr
+    out = torch.empty((20, n), dtype=torch.float64, device=gpu).t()
+    Ys = dev.matvec_batched(Xw[:, 5:25], precision=precision, out=out)
+    assert Ys.data_ptr() == out.data_ptr()
+    assert np.array_equal(Ys.cpu().numpy(), ref)
+    # the adjoint through the same entry point
+    Yt = dev.rmatvec_batched(Xr, precision=precision).cpu().numpy()
+    tol = TOL64 if precision == "fp64" else TOL32
+    for j in (0, 19):
+        assert rel(Yt[:, j], O.rmatvec_original(h.flat(), Xh[:, j])) <= tol
+    # overlapping output strides and a zero stride are refused
+    Ybad = torch.empty((n, 20), dtype=torch.float64, device=gpu)
+    for ys in ((1, 1), (2, 1), (0, 1)):
+        rc = N.hip().bw_hip_matvec_batched_strided(
+            dev._h, 0, 20, C.c_void_p(Xr.data_ptr()), 20, 1, C.c_void_p(Ybad.data_ptr()),
+            ys[0], ys[1], N.BW_ORDER_ORIGINAL, prec, C.c_void_p(stream))
+        assert rc == N.BW_ESHAPE, ys

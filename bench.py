@@ -332,7 +332,8 @@ def bench_batching(args, cfg):
     tuning = None
     if not args.no_autotune:
         from bigwham_dev_amd import autotune
-        tuning = autotune.tune_mfma(dev, k)
+        tuning = {"fp32": autotune.tune_mfma(dev, k, precision="fp32"),
+                  "fp64": autotune.tune_mfma(dev, k, precision="fp64")}
         log(f"mfma schedules (ms per 32-RHS call): {tuning}")
     t_b32 = _timed(lambda: dev.matvec_batched(X, precision="fp32"), steps, warmup)
     x1 = X[:, 0].contiguous()

@@ -108,7 +108,7 @@ def hip() -> C.CDLL:
         lib.bw_hip_set_tuning.argtypes = [C.c_int64, C.c_int64]
         lib.bw_hip_set_stacking.argtypes = [C.c_int]
         lib.bw_hip_set_unroll.argtypes = [_vp, C.c_int, C.c_int]
-        lib.bw_hip_set_mfma_schedule.argtypes = [_vp, C.c_int]
+        lib.bw_hip_set_mfma_schedule.argtypes = [_vp, C.c_int, C.c_int]
         lib.bw_hip_set_timing.argtypes = [_vp, C.c_int]
         lib.bw_hip_timing_stats.argtypes = [_vp, _d, _i64]
         lib.bw_hip_destroy.argtypes = [_vp]

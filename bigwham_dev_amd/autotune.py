@@ -4,8 +4,9 @@ Three knobs shape a matvec's schedule on the device:
 
 - the unroll depth of ``k_slab`` (loads in flight per lane: 4, 8 or 16),
   switchable per handle at no cost (``bw_hip_set_unroll``);
-- the schedule of the FP32 MFMA kernel of the batched matvec (group pairs per
-  step, loads pipelined or not; ``bw_hip_set_mfma_schedule``), also free;
+- the schedule of the MFMA kernels of the batched matvec (group pairs per
+  step, depth of the load pipeline; ``bw_hip_set_mfma_schedule``, per
+  precision), also free;
 - the work-item size (bytes of factors one workgroup streams), fixed when the
   layout is built (``bw_hip_set_tuning`` before ``HipHmat``).
 
@@ -25,7 +26,8 @@ import os
 from dataclasses import dataclass, field
 
 UNROLLS = (4, 8, 16)
-MFMA_SCHEDULES = (0, 1, 3, 4, 5, 6, 7)
+MFMA_SCHEDULES = (0, 1, 3, 4, 5, 6, 7)   # 2 spills registers on gfx950
+MFMA64_SCHEDULES = (0, 4, 5, 6, 7)       # 1-3 are 0 for FP64
 WI_BYTES = (64 << 10, 96 << 10, 192 << 10, 384 << 10)
 
 
@@ -105,26 +107,28 @@ def tune_unroll(dev, precisions=None, reps: int = 5, use_cache: bool = True) -> 
     return res
 
 
-def tune_mfma(dev, nrhs: int = 32, reps: int = 5) -> dict:
-    """Time the schedules of the FP32 MFMA batched kernel (group pairs per
-    step, pipelined or not) on `nrhs` right-hand sides and keep the fastest.
-    Returns {schedule: ms per batched call, "best": schedule}."""
+def tune_mfma(dev, nrhs: int = 32, reps: int = 5, precision: str = "fp32") -> dict:
+    """Time the schedules of the batched MFMA kernel of one precision (group
+    pairs per step, depth of the load pipeline) on `nrhs` right-hand sides
+    and keep the fastest.  Returns {schedule: ms per batched call, "best":
+    schedule}."""
     import time
 
     import torch
     X = torch.ones((dev.n_cols, nrhs), dtype=torch.float64, device=dev.torch_device)
+    Y = torch.empty((dev.n_rows, nrhs), dtype=torch.float64, device=dev.torch_device)
     ms = {}
-    for sch in MFMA_SCHEDULES:
-        dev.set_mfma_schedule(sch)
-        dev.matvec_batched(X, precision="fp32")
+    for sch in (MFMA_SCHEDULES if precision == "fp32" else MFMA64_SCHEDULES):
+        dev.set_mfma_schedule(sch, precision)
+        dev.matvec_batched(X, precision=precision, out=Y)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(reps):
-            dev.matvec_batched(X, precision="fp32")
+            dev.matvec_batched(X, precision=precision, out=Y)
         torch.cuda.synchronize()
         ms[sch] = (time.perf_counter() - t0) / reps * 1e3
     best = min(ms, key=ms.get)
-    dev.set_mfma_schedule(best)
+    dev.set_mfma_schedule(best, precision)
     return {**ms, "best": best}
 
 

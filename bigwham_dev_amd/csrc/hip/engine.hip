@@ -613,8 +613,12 @@ __global__ __launch_bounds__(256) void k_reduce_b(const bwl::Reduce* __restrict_
 // past the end get B = 0.
 typedef double dblx4 __attribute__((ext_vector_type(4)));
 
-template <int PASS>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_slab_mfma64(
+// D > 0: the k-steps run through a ring of D register buffers (as in
+// mfma_stream_deep), D - 1 steps of loads in flight; D = 0: two steps loaded,
+// then their MFMAs.  Both issue the same MFMAs in the same order per
+// accumulator, so the results agree bit for bit.
+template <int PASS, int D>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(D > 4 ? 2 : (D > 0 ? 3 : 4)))) void k_slab_mfma64(
     const bwl::WorkItem* __restrict__ wis, XcdSplit xs, const int32_t* __restrict__ gidx,
     const double* __restrict__ data, const double* __restrict__ XB,
     double* __restrict__ tout,             // pass 1
@@ -638,6 +642,50 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   int rowc[4];
 #pragma unroll
   for (int t = 0; t < 4; ++t) rowc[t] = min(16 * t + i16, R - 1);
+  if constexpr (D > 0) {
+    const int nsteps = (ng + 1) >> 1;  // two groups per k-step
+    double a[D][4], b[D][2];
+    int zz[D];  // input offset of the next step to be loaded into each buffer
+    auto gz = [&](int step) { return gi[min(2 * step + gsub, ng - 1)]; };
+    auto load = [&](int step, int z, double (&aa)[4], double (&bb)[2]) {
+      const int ga = min(2 * step + gsub, ng - 1);
+      const double* __restrict__ gp = base + static_cast<int64_t>(ga) * R * 2 + v;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) aa[t] = t < ntile ? gp[rowc[t] * 2] : 0.0;
+      const int64_t zr = static_cast<int64_t>(z + v) * kRhs;
+      bb[0] = XB[zr + i16];
+      bb[1] = XB[zr + 16 + i16];
+    };
+    auto compute = [&](int step, const double (&aa)[4], const double (&bb)[2]) {
+      const bool ok = 2 * step + gsub < ng;
+      const double b0 = ok ? bb[0] : 0.0, b1 = ok ? bb[1] : 0.0;
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+        if (t < ntile) {
+          acc[t][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(aa[t], b0, acc[t][0], 0, 0, 0);
+          acc[t][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(aa[t], b1, acc[t][1], 0, 0, 0);
+        }
+    };
+#pragma unroll
+    for (int q = 0; q < D; ++q) zz[q] = gz(q);
+#pragma unroll
+    for (int q = 0; q < D - 1; ++q) {
+      load(q, zz[q], a[q], b[q]);
+      zz[q] = gz(q + D);
+    }
+    for (int s0 = 0; s0 < nsteps; s0 += D) {
+#pragma unroll
+      for (int j = 0; j < D; ++j) {
+        const int st = s0 + j;
+        const int q = (j + D - 1) % D;  // buffer freed by the previous stage
+        load(st + D - 1, zz[q], a[q], b[q]);
+        zz[q] = gz(st + 2 * D - 1);
+        __builtin_amdgcn_sched_barrier(0);
+        if (st < nsteps) compute(st, a[j], b[j]);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+  } else {
   constexpr int S = 2;  // k-steps (two groups each) loaded before their MFMAs
   for (int g0 = 0; g0 < ng; g0 += 2 * S) {
     double a[S][4], b[S][2];
@@ -663,6 +711,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
           acc[t][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s][t], b[s][1], acc[t][1], 0, 0, 0);
         }
   }
+  }
   double* pp = wi.part_off >= 0 ? part + static_cast<int64_t>(wi.part_off) * kRhs : nullptr;
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
@@ -685,6 +734,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         }
       }
   }
+}
+
+template <int PASS>
+void launch_mfma64(int schedule, int blocks, hipStream_t st, const bwl::WorkItem* wis, XcdSplit xs,
+                   const int32_t* gidx, const double* data, const double* XB, double* tout,
+                   double* Y, int64_t yr, int64_t yc, int nrhs, double* part,
+                   const int64_t* perm, int dof, const int32_t* rowmap) {
+#define BW_MFMA64(D)                                                                        \
+  hipLaunchKernelGGL((k_slab_mfma64<PASS, D>), dim3(blocks), dim3(256), 0, st, wis, xs, gidx, \
+                     data, XB, tout, Y, yr, yc, nrhs, part, perm, dof, rowmap)
+  switch (schedule) {
+    case 4: BW_MFMA64(3); break;
+    case 5: BW_MFMA64(4); break;
+    case 6: BW_MFMA64(6); break;
+    case 7: BW_MFMA64(8); break;
+    default: BW_MFMA64(0);
+  }
+#undef BW_MFMA64
 }
 
 // ---------------------------------------------------------------------------
@@ -714,7 +781,7 @@ struct DevOp {
   int64_t stream_bytes = 0;
   int64_t part_elems[2] = {0, 0};
   int unroll = 16;            // loads in flight per lane in k_slab (4, 8 or 16)
-  int mfma_schedule = 0;      // k_slab_mfma variant (0-7)
+  int mfma_schedule = 0;      // k_slab_mfma / k_slab_mfma64 variant (0-7)
   // many right-hand sides (FP32 MFMA path), allocated on first use
   void* xb = nullptr;        // xt_size x kRhs
   void* partb[2] = {nullptr, nullptr};
@@ -1513,16 +1580,16 @@ int batched_impl(bw_hip_handle* hd, bool adjoint, int nrhs, const double* X, int
       hipLaunchKernelGGL(k_gather_b<double>, dim3(gblocks), dim3(256), 0, st, Xk, xr, xc, nk, xb,
                          op.n_in, pin, dof);
       if (op.p1.nwi > 0)
-        hipLaunchKernelGGL(k_slab_mfma64<1>, dim3(op.p1.blocks), dim3(256), 0, st, op.p1.wis,
-                           op.p1.xs, op.p1.gidx, static_cast<const double*>(op.p1.data), xb, xb,
-                           nullptr, 0, 0, kRhs, pb0, nullptr, dof, op.p1.rowmap);
+        launch_mfma64<1>(op.mfma_schedule, op.p1.blocks, st, op.p1.wis, op.p1.xs, op.p1.gidx,
+                         static_cast<const double*>(op.p1.data), xb, xb, nullptr, 0, 0, kRhs,
+                         pb0, nullptr, dof, op.p1.rowmap);
       if (op.p1.nred > 0)
         hipLaunchKernelGGL((k_reduce_b<1, double>), dim3(op.p1.nred), dim3(256), 0, st, op.p1.reds,
                            op.p1.nred, pb0, xb, nullptr, 0, 0, kRhs, nullptr, dof, op.p1.rowmap);
       if (op.p2.nwi > 0)
-        hipLaunchKernelGGL(k_slab_mfma64<2>, dim3(op.p2.blocks), dim3(256), 0, st, op.p2.wis,
-                           op.p2.xs, op.p2.gidx, static_cast<const double*>(op.p2.data), xb,
-                           nullptr, Yk, yr, yc, nk, pb1, pout, dof, nullptr);
+        launch_mfma64<2>(op.mfma_schedule, op.p2.blocks, st, op.p2.wis, op.p2.xs, op.p2.gidx,
+                         static_cast<const double*>(op.p2.data), xb, nullptr, Yk, yr, yc, nk,
+                         pb1, pout, dof, nullptr);
       if (op.p2.nred > 0)
         hipLaunchKernelGGL((k_reduce_b<2, double>), dim3(op.p2.nred), dim3(256), 0, st, op.p2.reds,
                            op.p2.nred, pb1, nullptr, Yk, yr, yc, nk, pout, dof, nullptr);
@@ -1609,11 +1676,16 @@ int bw_hip_set_unroll(bw_hip_handle* hd, int unroll_fp64, int unroll_fp32) {
   return BW_OK;
 }
 
-int bw_hip_set_mfma_schedule(bw_hip_handle* hd, int schedule) {
+int bw_hip_set_mfma_schedule(bw_hip_handle* hd, int precision_mask, int schedule) {
   if (!hd) return fail(BW_EINVAL, "set_mfma_schedule: null handle");
   std::lock_guard<std::mutex> lock(hd->mu);
   if (schedule < 0 || schedule > 7) return fail(BW_EINVAL, "set_mfma_schedule: schedule must be 0-7");
-  for (int adj = 0; adj < 2; ++adj) hd->op[adj][1].mfma_schedule = schedule;
+  if (precision_mask <= 0 || (precision_mask & ~(BW_PREC_FP64 | BW_PREC_FP32)))
+    return fail(BW_EINVAL, "set_mfma_schedule: bad precision mask");
+  for (int adj = 0; adj < 2; ++adj) {
+    if (precision_mask & BW_PREC_FP64) hd->op[adj][0].mfma_schedule = schedule;
+    if (precision_mask & BW_PREC_FP32) hd->op[adj][1].mfma_schedule = schedule;
+  }
   return BW_OK;
 }
 

@@ -339,8 +339,10 @@ class HipHmat:
     def set_unroll(self, fp64: int = 0, fp32: int = 0) -> None:
         N.check_hip(N.hip().bw_hip_set_unroll(self._h, int(fp64), int(fp32)))
 
-    def set_mfma_schedule(self, schedule: int) -> None:
-        N.check_hip(N.hip().bw_hip_set_mfma_schedule(self._h, int(schedule)))
+    def set_mfma_schedule(self, schedule: int, precision: str = "fp32") -> None:
+        """Schedule (0-7) of the batched MFMA kernel of one precision."""
+        N.check_hip(N.hip().bw_hip_set_mfma_schedule(self._h, self._prec(precision),
+                                                     int(schedule)))
 
     def set_timing(self, enable: bool = True) -> None:
         N.check_hip(N.hip().bw_hip_set_timing(self._h, int(enable)))

@@ -172,13 +172,15 @@ int bw_hip_set_stacking(int stack_rows);
  * 0 keeps the current value).  Chosen by bigwham_dev_amd.autotune. */
 int bw_hip_set_unroll(bw_hip_handle* hd, int unroll_fp64, int unroll_fp32);
 
-/* Schedule of the FP32 MFMA kernel of bw_hip_matvec_batched: group pairs per
- * step and whether the next step's loads are issued before the current
- * step's MFMAs (0: 4 pairs; 1: 2 pairs, pipelined; 2: 4 pairs, pipelined;
- * 3: 8 pairs; 4-7: a ring of D step buffers with D - 1 steps of loads in
- * flight: 4: 2 pairs, D = 3; 5: 2 pairs, D = 4; 6: 1 pair, D = 6; 7: 1 pair,
- * D = 8).  Chosen by bigwham_dev_amd.autotune. */
-int bw_hip_set_mfma_schedule(bw_hip_handle* hd, int schedule);
+/* Schedule of the MFMA kernels of the batched products, for the precisions
+ * in precision_mask.  FP32: group pairs per step and whether the next step's
+ * loads are issued before the current step's MFMAs (0: 4 pairs; 1: 2 pairs,
+ * pipelined; 2: 4 pairs, pipelined; 3: 8 pairs; 4-7: a ring of D step
+ * buffers with D - 1 steps of loads in flight: 4: 2 pairs, D = 3; 5: 2 pairs,
+ * D = 4; 6: 1 pair, D = 6; 7: 1 pair, D = 8).  FP64: 0-3 load two k-steps
+ * before their MFMAs; 4-7 the ring with D = 3, 4, 6, 8.  Every schedule gives
+ * the same bits.  Chosen by bigwham_dev_amd.autotune. */
+int bw_hip_set_mfma_schedule(bw_hip_handle* hd, int precision_mask, int schedule);
 
 /* Per-kernel timing.  With timing enabled every matvec records HIP events
  * between its kernels on the call's stream; bw_hip_timing_stats returns the

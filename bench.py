@@ -117,6 +117,42 @@ def cpu_baseline(host, x, budget_s=15.0):
                           f"{med * 1e3:.2f} ms, OpenMP restatement of Hmat::matvecOriginal")
 
 
+def dense_check(host, dev, x, max_dense=12000, blocks=8, m=8):
+    """Matvec error against the exact dense operator (toArray2D,
+    matrix_generator.h:35-40, evaluated by the host kernels): the whole
+    operator when N <= max_dense, else `blocks` random blocks of m
+    consecutive points' rows (permuted numbering).  The gap measures the ACA
+    approximation, not the arithmetic.  Returns (summary, dense or None)."""
+    import torch
+    n, dof = host.size, host.dof
+    dev0 = "cuda:0"
+    if n <= max_dense:
+        D = host.dense(original=True)
+        y_ex = D @ x
+        xd = torch.from_numpy(x).to(dev0)
+        got = {p: dev.matvec(xd, original=True, precision=p).cpu().numpy()
+               for p in ("fp64", "fp32")}
+        rows = n
+    else:
+        D = None
+        perm = host.permutation()
+        idx = (dof * perm[:, None] + np.arange(dof)[None, :]).ravel()
+        xp = x[idx]  # permuted numbering: z = x gathered through permutation_1_
+        xd = torch.from_numpy(xp).to(dev0)
+        yp = {p: dev.matvec(xd, original=False, precision=p).cpu().numpy()
+              for p in ("fp64", "fp32")}
+        npts = n // dof
+        starts = np.sort(np.random.default_rng(99).choice(npts - m, blocks, replace=False))
+        sl = np.concatenate([np.arange(dof * b, dof * (b + m)) for b in starts])
+        y_ex = np.concatenate([host.dense_block(int(b), 0, m, npts) @ xp for b in starts])
+        got = {p: yp[p][sl] for p in yp}
+        rows = int(sl.size)
+    nrm = np.linalg.norm(y_ex)
+    return ({"rows": rows, "of": n, "sampled": D is None,
+             "fp64_rel_err": float(np.linalg.norm(got["fp64"] - y_ex) / nrm),
+             "fp32_rel_err": float(np.linalg.norm(got["fp32"] - y_ex) / nrm)}, D)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -263,6 +299,11 @@ def main():
         }
         out["vs_cpu"] = round(out["value"] / cb["value"], 2)
 
+    t0 = time.perf_counter()
+    out["vs_dense_operator"], D = dense_check(host, dev, x_np)
+    out["vs_dense_operator"]["seconds"] = round(time.perf_counter() - t0, 2)
+    log(f"vs dense operator {out['vs_dense_operator']}")
+
     if args.gmres:
         b = torch.zeros(n, dtype=torch.float64, device="cuda:0")
         b[host.dof - 1::host.dof] = 1.0  # unit normal pressure, local frame (normal last)
@@ -281,6 +322,11 @@ def main():
             "solution_rel_diff": float(torch.linalg.vector_norm(xm - x64) /
                                        torch.linalg.vector_norm(x64)),
         }
+        if D is not None:  # small N: the dense FP64 solve of the exact operator
+            xs = np.linalg.solve(D, b.cpu().numpy())
+            for key, sol in (("fp64", x64), ("mixed", xm)):
+                out["gmres"][key]["rel_err_vs_dense_solve"] = float(
+                    np.linalg.norm(sol.cpu().numpy() - xs) / np.linalg.norm(xs))
         if "cpu_baseline" in out:
             # the host path would run the same FP64 GMRES (scipy on BEMatrix) with
             # one OpenMP matvec per iteration: bounded by its matvec count times

@@ -274,10 +274,15 @@ class HipHmat:
                adjoint: bool = False):
         """y = H x (or H^T x).  numpy in -> numpy out (host staging, like
         HmatCuda::matvec(ArrayView)); torch CUDA tensor in -> tensor out."""
+        n_in, n_out = (self.n_rows, self.n_cols) if adjoint else (self.n_cols, self.n_rows)
         if isinstance(x, np.ndarray):
             x = np.ascontiguousarray(x, dtype=np.float64)
-            n_out = self.n_cols if adjoint else self.n_rows
+            if x.size != n_in:
+                raise ValueError(f"matvec: x has {x.size} entries, the operator takes {n_in}")
             y = np.empty(n_out) if out is None else out
+            if (not isinstance(y, np.ndarray) or y.dtype != np.float64 or y.size != n_out
+                    or not y.flags.c_contiguous):
+                raise ValueError(f"matvec: out must be a contiguous float64 array of {n_out}")
             fn = N.hip().bw_hip_rmatvec if adjoint else N.hip().bw_hip_matvec
             N.check_hip(fn(self._h, x.ctypes.data_as(C.c_void_p), y.ctypes.data_as(C.c_void_p),
                            N.BW_ORDER_ORIGINAL if original else N.BW_ORDER_PERMUTED,
@@ -287,8 +292,13 @@ class HipHmat:
         if not (isinstance(x, torch.Tensor) and x.is_cuda and x.dtype == torch.float64):
             raise TypeError("x must be a numpy array or a float64 CUDA tensor")
         x = x.contiguous()
-        n_out = self.n_cols if adjoint else self.n_rows
+        if x.numel() != n_in:
+            raise ValueError(f"matvec: x has {x.numel()} entries, the operator takes {n_in}")
         y = torch.empty(n_out, dtype=torch.float64, device=x.device) if out is None else out
+        if (not isinstance(y, torch.Tensor) or y.dtype != torch.float64 or y.device != x.device
+                or y.numel() != n_out or not y.is_contiguous()):
+            raise ValueError(f"matvec: out must be a contiguous float64 tensor of {n_out} "
+                             "on the device of x")
         stream = torch.cuda.current_stream(x.device).cuda_stream
         self.matvec_ptr(x.data_ptr(), y.data_ptr(), original, precision, stream, adjoint)
         return y
@@ -369,9 +379,15 @@ class MatvecGraph:
         if not (isinstance(x, torch.Tensor) and x.is_cuda and x.dtype == torch.float64
                 and x.is_contiguous()):
             raise TypeError("x must be a contiguous float64 CUDA tensor")
-        n_out = dev.n_cols if adjoint else dev.n_rows
+        n_in, n_out = (dev.n_rows, dev.n_cols) if adjoint else (dev.n_cols, dev.n_rows)
+        if x.numel() != n_in:
+            raise ValueError(f"graph: x has {x.numel()} entries, the operator takes {n_in}")
         if y is None:
             y = torch.empty(n_out, dtype=torch.float64, device=x.device)
+        if (not isinstance(y, torch.Tensor) or y.dtype != torch.float64 or y.device != x.device
+                or y.numel() != n_out or not y.is_contiguous()):
+            raise ValueError(f"graph: y must be a contiguous float64 tensor of {n_out} "
+                             "on the device of x")
         self.dev, self.x, self.y = dev, x, y
         out = C.c_void_p()
         N.check_hip(N.hip().bw_hip_graph_create(

@@ -80,3 +80,31 @@ def test_batched_single_column_and_tiny_selection(gpu):
     one = HipHmat.many([h], precisions=("fp64",))
     xd = X[:, 0].contiguous()
     assert rel(one.matvec(xd).cpu().numpy(), ref) <= 1e-12
+
+
+def test_wrong_lengths_are_refused(gpu):
+    """Vectors of the wrong length or type never reach a kernel."""
+    import torch
+    coor, conn = rect_grid(6)
+    h = HostHmat.from_mesh("3DR0-H", coor, conn, [1.0, 0.25])
+    h.build(64, 3.0, 1e-6)
+    dev = HipHmat(h, precisions=("fp64",))
+    n = h.size
+    print(f"wrong lengths n={n}", flush=True)
+    for bad in (np.zeros(n - 1), np.zeros(n + 3)):
+        with pytest.raises(ValueError):
+            dev.matvec(bad)
+        with pytest.raises(ValueError):
+            dev.matvec(torch.from_numpy(bad).to(gpu))
+    with pytest.raises(ValueError):
+        dev.matvec(np.zeros(n), out=np.zeros(n - 1))
+    with pytest.raises(ValueError):
+        dev.matvec(torch.zeros(n, dtype=torch.float64, device=gpu),
+                   out=torch.zeros(n, dtype=torch.float32, device=gpu))
+    with pytest.raises(ValueError):
+        dev.graph(torch.zeros(n - 1, dtype=torch.float64, device=gpu))
+    with pytest.raises(ValueError):
+        dev.matvec_batched(torch.zeros((n - 1, 4), dtype=torch.float64, device=gpu))
+    # and the right lengths still work afterwards
+    x = np.random.default_rng(3).uniform(-1, 1, n)
+    assert rel(dev.matvec(x), O.matvec_original(h.flat(), x)) <= 1e-12

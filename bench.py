@@ -242,6 +242,22 @@ def main():
     y64 = y.cpu().numpy()
     dt32, st32 = run("fp32", args.steps, args.warmup)
     y32 = y.cpu().numpy()
+
+    # the same matvecs replayed from one captured HIP graph each (what a
+    # solver loop on fixed vectors can use): removes the per-call host and
+    # launch overhead, which matters for the small matrices
+    graph_ms = {}
+    for prec in ("fp64", "fp32"):
+        g = dev.graph(x, y, precision=prec)
+        for _ in range(args.warmup):
+            g.launch()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            g.launch()
+        torch.cuda.synchronize()
+        graph_ms[prec] = round((time.perf_counter() - t0) / args.steps * 1e3, 4)
+        g.close()
     log(f"fp64 {dt64 / args.steps * 1e3:.3f} ms/matvec, fp32 {dt32 / args.steps * 1e3:.3f} ms/matvec")
     log(f"fp64 kernels {st64}")
     log(f"fp32 kernels {st32}")
@@ -286,6 +302,7 @@ def main():
         "roofline": rf64,
         "fp32": {"value": round(args.steps / dt32, 3), "unit": "matvec/s",
                  "ms_per_step": round(dt32 / args.steps * 1e3, 4), "roofline": rf32},
+        "graph_replay_ms_per_step": graph_ms,
         "setup_s": {"host_build": round(t_build, 2), "layout_upload": round(t_upload, 2)},
         "tuning": tuning,
     }

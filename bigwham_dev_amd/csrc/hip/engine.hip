@@ -131,27 +131,17 @@ __global__ __launch_bounds__(256) void k_slab(const bwl::WorkItem* __restrict__ 
   int split = 0;
   if (R > 32) {
     // One column group per iteration for the whole wavefront: group index
-    // and input entries are wave-uniform (scalar loads).  The group indices
-    // of a batch are loaded one batch ahead, so its input entries go out with
-    // its factor loads; the last, partial batch (fewer than U groups) is one
-    // more batch of clamped, in-bounds loads whose surplus groups are skipped,
-    // rather than a loop of one dependent round trip per group.
+    // and input entries are wave-uniform (scalar loads).
     if (lane < R) {
       const VT* p = base + lane;
-      int zc[U];
-#pragma unroll
-      for (int u = 0; u < U; ++u) zc[u] = gi[min(u, ng - 1)];
       int g = 0;
       for (; g + U <= ng; g += U) {
         VT a[U];
-        int zn[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) a[u] = __builtin_nontemporal_load(p + (g + u) * R);
 #pragma unroll
-        for (int u = 0; u < U; ++u) zn[u] = gi[min(g + U + u, ng - 1)];
-#pragma unroll
         for (int u = 0; u < U; ++u) {
-          const int z = __builtin_amdgcn_readfirstlane(zc[u]);
+          const int z = __builtin_amdgcn_readfirstlane(gi[g + u]);
 #pragma unroll
           for (int v = 0; v < V; ++v) {
             if (u & 1)
@@ -160,21 +150,24 @@ __global__ __launch_bounds__(256) void k_slab(const bwl::WorkItem* __restrict__ 
               acc0 += a[u][v] * xin[z + v];
           }
         }
-#pragma unroll
-        for (int u = 0; u < U; ++u) zc[u] = zn[u];
       }
-      if (g < ng) {
-        VT a[U];
+      // the last, partial batch in steps of four groups, then one at a time
+      for (; g + 4 <= ng; g += 4) {
+        VT a[4];
 #pragma unroll
-        for (int u = 0; u < U; ++u) a[u] = __builtin_nontemporal_load(p + min(g + u, ng - 1) * R);
+        for (int u = 0; u < 4; ++u) a[u] = __builtin_nontemporal_load(p + (g + u) * R);
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-          if (g + u < ng) {
-            const int z = __builtin_amdgcn_readfirstlane(zc[u]);
+        for (int u = 0; u < 4; ++u) {
+          const int z = __builtin_amdgcn_readfirstlane(gi[g + u]);
 #pragma unroll
-            for (int v = 0; v < V; ++v) acc0 += a[u][v] * xin[z + v];
-          }
+          for (int v = 0; v < V; ++v) acc0 += a[u][v] * xin[z + v];
         }
+      }
+      for (; g < ng; ++g) {
+        const VT a = __builtin_nontemporal_load(p + g * R);
+        const int z = __builtin_amdgcn_readfirstlane(gi[g]);
+#pragma unroll
+        for (int v = 0; v < V; ++v) acc0 += a[v] * xin[z + v];
       }
     }
   } else {
@@ -217,25 +210,11 @@ __global__ __launch_bounds__(256) void k_slab(const bwl::WorkItem* __restrict__ 
 #pragma unroll
         for (int u = 0; u < U; ++u) zc[u] = zn[u];
       }
-      // the remaining groups of this split (fewer than U): one batch of
-      // clamped loads, accumulated in group order
-      if (g < ng) {
-        VT a[U];
-        T xv[U][V];
+      for (; g < ng; g += nsplit) {
+        const VT a = __builtin_nontemporal_load(p + g * R);
+        const int z = gi[g];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-          const int gu = min(g + u * nsplit, ng - 1);
-          a[u] = __builtin_nontemporal_load(p + gu * R);
-          const int z = gi[gu];
-#pragma unroll
-          for (int v = 0; v < V; ++v) xv[u][v] = xin[z + v];
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u)
-          if (g + u * nsplit < ng) {
-#pragma unroll
-            for (int v = 0; v < V; ++v) acc0 += a[u][v] * xv[u][v];
-          }
+        for (int v = 0; v < V; ++v) acc0 += a[v] * xin[z + v];
       }
     }
   }

@@ -170,7 +170,7 @@ def main():
                     help="keep the default unroll depth instead of timing 4/8/16 first")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU only; multi-GPU sharding is a design "
+        print("bench.py: bigwham_dev_amd runs on one GPU only; multi-GPU sharding is a design "
               "note (block-row ownership + RCCL all-gather of x), not a measured path.",
               file=sys.stderr)
         sys.exit(2)

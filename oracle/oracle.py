@@ -1,7 +1,7 @@
 """ctypes wrapper of the CPU oracle (oracle/lib/liboracle.so).
 
 TEST INFRASTRUCTURE ONLY.  Imported by tests/, __graft_entry__.smoke() and
-bench.py's cpu_baseline leg, as the checker and as the host-core baseline;
+bench.py's cpu_baseline leg, as the expected result and as the host-core baseline;
 never by the bigwham_dev_amd package.  Every function states the reference
 routine it restates (see hmat_oracle.c).
 """

@@ -15,8 +15,8 @@ def pytest_configure(config):
 
 
 def _ensure_built():
-    """Build missing libraries on the CPU side only (never inside a GPU run:
-    the GPU box receives the prebuilt .so files)."""
+    """Build the libraries only if they are missing (a fresh checkout on a
+    build machine); GPU test runs are expected to find them prebuilt."""
     from bigwham_dev_amd import build as b
     if not (b.LIB / "libbwhost.so").exists():
         b.build_host()

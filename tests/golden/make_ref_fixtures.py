@@ -5,7 +5,7 @@ pattern and segment-kernel sources by oracle/ref/build_ref.sh).
 Stored per mesh and clustering variant: the collocation points given to the
 reference, its permutation and its FRB / LRB pattern tables (rows 0-4);
 plus We_segment_0 outputs on 20 seeded receivers.  tests/test_golden.py checks
-this build's host library against them, so the pins hold where the reference
+the host library of bigwham_dev_amd against them, so the pins hold where the reference
 sources are not mounted.  Run: python tests/golden/make_ref_fixtures.py"""
 from __future__ import annotations
 

@@ -1,4 +1,4 @@
-"""This build's host library against fixtures dumped from the compiled
+"""The host library of bigwham_dev_amd against fixtures dumped from the compiled
 reference harness (tests/golden/make_ref_fixtures.py, oracle/_ref built from
 the reference's clustering, pattern and segment-kernel sources), so the pins
 hold where the reference sources are not mounted.  CPU only."""

@@ -1,7 +1,7 @@
 """The host-side C++ (H construction, slab layout, composite, full blocks)
 built with g++ -fsanitize=address,undefined and driven through its C-ABI
 (tests/asan/host_driver.cpp): any out-of-bounds access, use after free, leak
-or undefined behaviour aborts the driver.  CPU only."""
+or undefined behaviour aborts the test program.  CPU only."""
 import os
 import shutil
 import subprocess

@@ -1,16 +1,16 @@
 """Parity at the benchmark sizes: the 3DR0-H 200x200 square crack
-(120 000 DoF) and the 3DR0-H-mode1 500x500 crack (250 000 DoF), FP64 against
-the oracle (restatement of Hmat::matvecOriginal, src/hmat/hmatrix/hmat.cc:
-572-715) at 1e-12, FP32 at 1e-5, plus size-independent properties: the
-adjoint identity, linearity and bitwise determinism.  The 600 000-DoF 3DT0-H
-penny matrix is checked the same way by bench.py (parity block of its JSON
-line)."""
+(120 000 DoF), the 3DR0-H-mode1 500x500 crack (250 000 DoF) and the 3DT0-H
+penny crack of ~200 000 triangles (599 739 DoF), FP64 against the oracle
+(restatement of Hmat::matvecOriginal, src/hmat/hmatrix/hmat.cc:572-715) at
+1e-12, FP32 at 1e-5, plus size-independent properties: the adjoint identity,
+linearity and bitwise determinism; and the 3DR0-H 224x224 crack (150 528 DoF)
+with 32 right-hand sides through the batched MFMA path."""
 import numpy as np
 import pytest
 
 import oracle as O
 from bigwham_dev_amd import HipHmat, HostHmat
-from bigwham_dev_amd.meshes import rect_grid
+from bigwham_dev_amd.meshes import penny_delaunay, rect_grid
 
 pytestmark = pytest.mark.gpu
 
@@ -44,3 +44,45 @@ def test_full_size(gpu, kernel, n):
     y2 = dev.matvec(2.0 * xd - 3.0 * w)
     assert float(torch.linalg.vector_norm(y2 - (2.0 * y - 3.0 * dev.matvec(w))) /
                  torch.linalg.vector_norm(y2)) <= 1e-13
+
+
+def test_full_size_penny(gpu):
+    """The 600k-DoF 3DT0-H penny matrix (FP64 + FP32 copies: 80 GB of HBM)."""
+    import torch
+    print("full size 3DT0-H penny, 105 000 points", flush=True)
+    coor, conn = penny_delaunay(105000, seed=0)
+    h = HostHmat.from_mesh("3DT0-H", coor, conn, [1.0, 0.25])
+    h.build(64, 3.0, 1e-6)
+    print(f"built N={h.size} E={h.nb_entries()}", flush=True)
+    dev = HipHmat(h, precisions=("fp64", "fp32"))
+    rng = np.random.default_rng(1234)
+    x = rng.uniform(-1, 1, h.size)
+    ref = O.matvec_original(h.flat(), x)
+    xd = torch.from_numpy(x).to(gpu)
+    y = dev.matvec(xd)
+    assert rel(y.cpu().numpy(), ref) <= 1e-12
+    assert torch.equal(y, dev.matvec(xd))
+    assert rel(dev.matvec(xd, precision="fp32").cpu().numpy(), ref) <= 1e-5
+    w = torch.from_numpy(rng.uniform(-1, 1, h.size)).to(gpu)
+    y2 = dev.matvec(2.0 * xd - 3.0 * w)
+    assert float(torch.linalg.vector_norm(y2 - (2.0 * y - 3.0 * dev.matvec(w))) /
+                 torch.linalg.vector_norm(y2)) <= 1e-13
+
+
+def test_full_size_batched(gpu):
+    """32 right-hand sides on the 224x224 3DR0-H crack, FP32 and FP64 MFMA,
+    against the oracle column by column (first, middle and last columns)."""
+    import torch
+    print("full size batched 3DR0-H 224x224 x 32", flush=True)
+    coor, conn = rect_grid(224)
+    h = HostHmat.from_mesh("3DR0-H", coor, conn, [1.0, 0.25])
+    h.build(64, 3.0, 1e-6)
+    dev = HipHmat(h, precisions=("fp64", "fp32"))
+    X = np.random.default_rng(7).uniform(-1, 1, (h.size, 32))
+    Xd = torch.from_numpy(X).to(gpu)
+    Y32 = dev.matvec_batched(Xd, precision="fp32").cpu().numpy()
+    Y64 = dev.matvec_batched(Xd, precision="fp64").cpu().numpy()
+    for j in (0, 15, 31):
+        ref = O.matvec_original(h.flat(), X[:, j])
+        assert rel(Y64[:, j], ref) <= 1e-12, j
+        assert rel(Y32[:, j], ref) <= 1e-5, j

@@ -996,6 +996,9 @@ int build_op(bw_hip_handle* hd, const bw_flat_hmat* h, bool adjoint, hipStream_t
   if ((rc = upload_pass<T>(L, L.pass1, &op.p1, st, &hd->allocs))) return rc;
   if ((rc = upload_pass<T>(L, L.pass2, &op.p2, st, &hd->allocs))) return rc;
   op.elem = sizeof(T);
+  // default batched schedule: the one the autotuner picked on MI355X for the
+  // 150k-DoF 3DR0 matrix (FP32: two pairs, ring of 3; FP64: ring of 4)
+  op.mfma_schedule = sizeof(T) == 8 ? 5 : 4;
   op.xt_size = L.xt_size;
   HIPCHK(hipMalloc(&op.xt, L.xt_size * sizeof(T)));
   hd->allocs.push_back(op.xt);

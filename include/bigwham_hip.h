@@ -179,7 +179,8 @@ int bw_hip_set_unroll(bw_hip_handle* hd, int unroll_fp64, int unroll_fp32);
  * buffers with D - 1 steps of loads in flight: 4: 2 pairs, D = 3; 5: 2 pairs,
  * D = 4; 6: 1 pair, D = 6; 7: 1 pair, D = 8).  FP64: 0-3 load two k-steps
  * before their MFMAs; 4-7 the ring with D = 3, 4, 6, 8.  Every schedule gives
- * the same bits.  Chosen by bigwham_dev_amd.autotune. */
+ * the same bits.  Default: 4 (FP32), 5 (FP64); bigwham_dev_amd.autotune
+ * times them all on the caller's matrix. */
 int bw_hip_set_mfma_schedule(bw_hip_handle* hd, int precision_mask, int schedule);
 
 /* Per-kernel timing.  With timing enabled every matvec records HIP events

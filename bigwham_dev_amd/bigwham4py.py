@@ -5,8 +5,10 @@ Same constructor arguments, same LinearOperator semantics (shape (N, N),
 float64, matvec in the original DoF ordering), same helper methods for the
 matvec path: getCompression, getPermutation, getCollocationPoints,
 getSpatialDimension, H_diag, H_jacobi_prec, H_ILU_prec, _getFullBlocks,
-get_diagonal, getMatvecTime, get_max_error_ACA, _getPattern, and
-sub-matrix selection by indexing (M[np.ix_(rows, cols)]).  Construction stays on the host (libbwhost);
+get_diagonal, getMatvecTime, get_max_error_ACA, _getPattern,
+write_hmatrix, get_element_normals, get_rotation_matrix,
+convert_to_global / convert_to_local, and sub-matrix selection by indexing
+(M[np.ix_(rows, cols)]).  Construction stays on the host (libbwhost);
 the matvec runs on the GPU (libbwhip).  `useCuda` is accepted for signature
 compatibility; there is no CPU matvec fallback in this package.
 
@@ -16,6 +18,7 @@ Extras of this engine: `precision="fp32"` for FP32-factor matvecs,
 """
 from __future__ import annotations
 
+import os
 import time
 
 import numpy as np
@@ -109,6 +112,45 @@ class BEMatrix(LinearOperator):
         return self.H_.collocation_points()
 
     getMeshCollocationPoints = getCollocationPoints
+
+    def get_omp_threads(self) -> int:
+        """Host threads of the H-matrix construction (the matvec runs on the GPU)."""
+        n = self.n_openMP_threads_
+        return n if n and n > 0 else (os.cpu_count() or 1)
+
+    def write_hmatrix(self, filename: str) -> int:
+        """Save the built operator (Hmat::writeToFile content; as a directory
+        of memory-mappable .npy arrays, see bigwham_dev_amd.store)."""
+        from . import store
+        self._build()
+        store.save(self.H_, filename)
+        return 0
+
+    def get_element_normals(self) -> np.ndarray:
+        """[n0x, n0y, (n0z), n1x, ...] as the reference returns them."""
+        return self.H_.element_normals().ravel()
+
+    def get_rotation_matrix(self) -> np.ndarray:
+        """Row-major rotation matrix of every element, concatenated."""
+        return self.H_.rotation_matrices().ravel()
+
+    def convert_to_global(self, x_local: np.ndarray) -> np.ndarray:
+        """Per-element vector from the element frame to the global frame,
+        v = R^T x (Mesh::ConvertToGlobal, src/core/mesh.h:91-106)."""
+        R = self.H_.rotation_matrices()
+        x = np.asarray(x_local, dtype=np.float64).reshape(R.shape[0], R.shape[1])
+        return np.einsum("eji,ej->ei", R, x).ravel()
+
+    def convert_to_local(self, x_global: np.ndarray) -> np.ndarray:
+        """Per-element vector from the global frame to the element frame,
+        v = R x (Mesh::ConvertToLocal, src/core/mesh.h:108-125)."""
+        R = self.H_.rotation_matrices()
+        x = np.asarray(x_global, dtype=np.float64).reshape(R.shape[0], R.shape[1])
+        return np.einsum("eij,ej->ei", R, x).ravel()
+
+    def isCudaAvailable(self) -> bool:
+        """True: the matvec of this engine always runs on the GPU."""
+        return True
 
     def getSpatialDimension(self) -> int:
         return self.H_.info()[1]

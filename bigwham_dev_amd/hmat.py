@@ -95,6 +95,21 @@ class HostHmat:
         N.check_host(N.host().bwh_collocation_points(self._h, N.dptr(out)))
         return out
 
+    def element_normals(self) -> np.ndarray:
+        """(n_elts, dim) unit normals (BigWhamIO::GetElementNormals)."""
+        _, dim, n = self.info()
+        out = np.empty(n * dim)
+        N.check_host(N.host().bwh_element_normals(self._h, N.dptr(out)))
+        return out.reshape(n, dim)
+
+    def rotation_matrices(self) -> np.ndarray:
+        """(n_elts, dim, dim) global-to-local rotations R of every element
+        (BigWhamIO::GetRotationMatrix): local = R @ global."""
+        _, dim, n = self.info()
+        out = np.empty(n * dim * dim)
+        N.check_host(N.host().bwh_rotation_matrices(self._h, N.dptr(out)))
+        return out.reshape(n, dim, dim)
+
     def permutation(self) -> np.ndarray:
         f = self.flat()
         return np.ctypeslib.as_array(f.perm1, shape=(f.n_col_pts,)).copy()

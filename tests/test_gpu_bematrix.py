@@ -77,3 +77,15 @@ def test_selection_by_indexing(gpu, M):
     assert fb.shape == S.shape and fb.nnz > 0
     with pytest.raises(NotImplementedError):
         S.H_diag()
+
+
+def test_write_hmatrix_replays(gpu, M, tmp_path):
+    """write_hmatrix saves the operator; the saved copy replays bit for bit."""
+    import torch
+    from bigwham_dev_amd import HipHmat, store
+    assert M.write_hmatrix(str(tmp_path / "h")) == 0
+    saved = store.load(tmp_path / "h")
+    dev = HipHmat(saved, precisions=("fp64",))
+    x = torch.from_numpy(np.random.default_rng(9).uniform(-1, 1, M.shape[0])).to(gpu)
+    assert torch.equal(dev.matvec(x), M.dev_.matvec(x))
+    assert M.isCudaAvailable() and M.get_omp_threads() >= 1

@@ -108,6 +108,7 @@ def hip() -> C.CDLL:
         lib.bw_hip_set_tuning.argtypes = [C.c_int64, C.c_int64]
         lib.bw_hip_set_stacking.argtypes = [C.c_int]
         lib.bw_hip_set_unroll.argtypes = [_vp, C.c_int, C.c_int]
+        lib.bw_hip_set_unroll_pass.argtypes = [_vp, C.c_int, C.c_int, C.c_int]
         lib.bw_hip_set_mfma_schedule.argtypes = [_vp, C.c_int, C.c_int]
         lib.bw_hip_set_timing.argtypes = [_vp, C.c_int]
         lib.bw_hip_timing_stats.argtypes = [_vp, _d, _i64]
@@ -174,7 +175,7 @@ HOST_SYMBOLS = ["bwh_create", "bwh_create_gaussian", "bwh_build_pattern", "bwh_b
                 "bwh_timings", "bwh_free", "bwh_last_error", "bwh_influence"]
 HIP_SYMBOLS = ["bw_hip_create", "bw_hip_matvec", "bw_hip_rmatvec", "bw_hip_matvec_batched",
                "bw_hip_rmatvec_batched", "bw_hip_matvec_batched_strided",
-               "bw_hip_diagonal", "bw_hip_sizes", "bw_hip_set_tuning", "bw_hip_set_stacking", "bw_hip_set_unroll", "bw_hip_set_mfma_schedule",
+               "bw_hip_diagonal", "bw_hip_sizes", "bw_hip_set_tuning", "bw_hip_set_stacking", "bw_hip_set_unroll", "bw_hip_set_unroll_pass", "bw_hip_set_mfma_schedule",
                "bw_hip_set_timing",
                "bw_hip_timing_stats", "bw_hip_destroy", "bw_hip_last_error",
                "bw_layout_build", "bw_layout_info", "bw_layout_export", "bw_layout_free",

@@ -11,7 +11,7 @@ Three knobs shape a matvec's schedule on the device:
   layout is built (``bw_hip_set_tuning`` before ``HipHmat``).
 
 ``tune_unroll`` times each unroll depth on the handle's own data with the
-engine's HIP-event timers and keeps the fastest one; ``tune_mfma`` does the
+engine's HIP-event timers and keeps the fastest one for each pass; ``tune_mfma`` does the
 same for the MFMA schedules.  ``tune_layout`` rebuilds
 the handle for each candidate work-item size (a full re-layout and upload, so
 only worth it for matrices that are applied many times) and returns the best.
@@ -33,7 +33,7 @@ WI_BYTES = (64 << 10, 96 << 10, 192 << 10, 384 << 10)
 
 @dataclass
 class TuneResult:
-    unroll: dict = field(default_factory=dict)      # precision -> best unroll
+    unroll: dict = field(default_factory=dict)      # precision -> [pass-1, pass-2] unroll
     ms: dict = field(default_factory=dict)          # precision -> {unroll: ms}
     wi_bytes: int = 0
 
@@ -69,8 +69,9 @@ def _store_cache(key: str, value: dict) -> None:
         pass
 
 
-def time_matvec(dev, x, y, precision: str, reps: int) -> float:
-    """Average device time (ms) of one matvec over ``reps`` calls."""
+def time_matvec(dev, x, y, precision: str, reps: int, stats: bool = False):
+    """Average device time (ms) of one matvec over ``reps`` calls (with
+    ``stats``, the dict of per-kernel averages)."""
     import torch
     dev.matvec(x, original=True, precision=precision, out=y)
     dev.set_timing(True)
@@ -79,31 +80,38 @@ def time_matvec(dev, x, y, precision: str, reps: int) -> float:
     torch.cuda.synchronize()
     st = dev.timing_stats()
     dev.set_timing(False)
-    return float(st["total"])
+    return st if stats else float(st["total"])
 
 
 def tune_unroll(dev, precisions=None, reps: int = 5, use_cache: bool = True) -> TuneResult:
-    """Pick the fastest unroll depth per precision and apply it to ``dev``."""
+    """Pick the fastest unroll depth of each pass, per precision, from one
+    timing run per depth (the engine's per-kernel timers), and apply them to
+    ``dev``.  ``unroll[p]`` is [pass 1, pass 2]; ``ms[p][u]`` the whole
+    matvec at depth u in both passes."""
     import torch
     precisions = tuple(precisions or dev.precisions)
-    key = "unroll:" + signature(dev)
+    key = "unroll2:" + signature(dev)
     res = TuneResult()
     cached = _load_cache().get(key) if use_cache else None
     if cached and all(p in cached for p in precisions):
-        res.unroll = {p: int(cached[p]) for p in precisions}
+        res.unroll = {p: [int(u) for u in cached[p]] for p in precisions}
     else:
         x = torch.ones(dev.n_cols, dtype=torch.float64, device=dev.torch_device)
         y = torch.empty(dev.n_rows, dtype=torch.float64, device=dev.torch_device)
         for p in precisions:
             res.ms[p] = {}
+            per_pass = ({}, {})
             for u in UNROLLS:
                 dev.set_unroll(**{p: u})
-                res.ms[p][u] = time_matvec(dev, x, y, p, reps)
-            res.unroll[p] = min(res.ms[p], key=res.ms[p].get)
+                st = time_matvec(dev, x, y, p, reps, stats=True)
+                res.ms[p][u] = float(st["total"])
+                per_pass[0][u], per_pass[1][u] = float(st["p1_slab"]), float(st["p2_slab"])
+            res.unroll[p] = [min(d, key=d.get) for d in per_pass]
         if use_cache:
             _store_cache(key, dict(res.unroll))
-    for p, u in res.unroll.items():
-        dev.set_unroll(**{p: u})
+    for p, (u1, u2) in res.unroll.items():
+        dev.set_unroll_pass(p, 1, u1)
+        dev.set_unroll_pass(p, 2, u2)
     return res
 
 

@@ -792,7 +792,7 @@ struct DevOp {
   const int64_t* perm_out = nullptr;  // device, n_out / dof
   int64_t stream_bytes = 0;
   int64_t part_elems[2] = {0, 0};
-  int unroll = 16;            // loads in flight per lane in k_slab (4, 8 or 16)
+  int unroll[2] = {16, 16};   // loads in flight per lane in k_slab (4, 8 or 16), per pass
   int mfma_schedule = 0;      // k_slab_mfma / k_slab_mfma64 variant (0-7)
   // many right-hand sides (FP32 MFMA path), allocated on first use
   void* xb = nullptr;        // xt_size x kRhs
@@ -1078,7 +1078,7 @@ int launch_matvec(bw_hip_handle* hd, DevOp& op, const double* xd, double* yd, in
   }
   if (hd->timing) hipEventRecord(hd->ev[1], st);
   if (op.p1.nwi > 0) {
-    launch_slab<T, T, 1>(op.unroll, op.p1.blocks, st, op.p1.wis, op.p1.xs, op.p1.gidx,
+    launch_slab<T, T, 1>(op.unroll[0], op.p1.blocks, st, op.p1.wis, op.p1.xs, op.p1.gidx,
                          static_cast<const T*>(op.p1.data), xt, xt, static_cast<T*>(op.p1.part),
                          nullptr, dof, op.p1.rowmap);
   }
@@ -1089,7 +1089,7 @@ int launch_matvec(bw_hip_handle* hd, DevOp& op, const double* xd, double* yd, in
                        op.p1.rowmap);
   if (hd->timing) hipEventRecord(hd->ev[3], st);
   if (op.p2.nwi > 0) {
-    launch_slab<T, double, 2>(op.unroll, op.p2.blocks, st, op.p2.wis, op.p2.xs, op.p2.gidx,
+    launch_slab<T, double, 2>(op.unroll[1], op.p2.blocks, st, op.p2.wis, op.p2.xs, op.p2.gidx,
                               static_cast<const T*>(op.p2.data), xt, yd,
                               static_cast<T*>(op.p2.part), orig ? op.perm_out : nullptr, dof,
                               nullptr);
@@ -1684,9 +1684,25 @@ int bw_hip_set_unroll(bw_hip_handle* hd, int unroll_fp64, int unroll_fp32) {
   for (int u : {unroll_fp64, unroll_fp32})
     if (u != 0 && u != 4 && u != 8 && u != 16)
       return fail(BW_EINVAL, "set_unroll: unroll must be 4, 8 or 16 (0 keeps the current one)");
+  for (int adj = 0; adj < 2; ++adj)
+    for (int pass = 0; pass < 2; ++pass) {
+      if (unroll_fp64) hd->op[adj][0].unroll[pass] = unroll_fp64;
+      if (unroll_fp32) hd->op[adj][1].unroll[pass] = unroll_fp32;
+    }
+  return BW_OK;
+}
+
+int bw_hip_set_unroll_pass(bw_hip_handle* hd, int precision_mask, int pass, int unroll) {
+  if (!hd) return fail(BW_EINVAL, "set_unroll_pass: null handle");
+  std::lock_guard<std::mutex> lock(hd->mu);
+  if (pass != 1 && pass != 2) return fail(BW_EINVAL, "set_unroll_pass: pass must be 1 or 2");
+  if (unroll != 4 && unroll != 8 && unroll != 16)
+    return fail(BW_EINVAL, "set_unroll_pass: unroll must be 4, 8 or 16");
+  if (precision_mask <= 0 || (precision_mask & ~(BW_PREC_FP64 | BW_PREC_FP32)))
+    return fail(BW_EINVAL, "set_unroll_pass: bad precision mask");
   for (int adj = 0; adj < 2; ++adj) {
-    if (unroll_fp64) hd->op[adj][0].unroll = unroll_fp64;
-    if (unroll_fp32) hd->op[adj][1].unroll = unroll_fp32;
+    if (precision_mask & BW_PREC_FP64) hd->op[adj][0].unroll[pass - 1] = unroll;
+    if (precision_mask & BW_PREC_FP32) hd->op[adj][1].unroll[pass - 1] = unroll;
   }
   return BW_OK;
 }

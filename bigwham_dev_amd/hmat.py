@@ -364,6 +364,11 @@ class HipHmat:
     def set_unroll(self, fp64: int = 0, fp32: int = 0) -> None:
         N.check_hip(N.hip().bw_hip_set_unroll(self._h, int(fp64), int(fp32)))
 
+    def set_unroll_pass(self, precision: str, pass_: int, unroll: int) -> None:
+        """Unroll depth of one pass (1 or 2) of one precision."""
+        N.check_hip(N.hip().bw_hip_set_unroll_pass(self._h, self._prec(precision), int(pass_),
+                                                   int(unroll)))
+
     def set_mfma_schedule(self, schedule: int, precision: str = "fp32") -> None:
         """Schedule (0-7) of the batched MFMA kernel of one precision."""
         N.check_hip(N.hip().bw_hip_set_mfma_schedule(self._h, self._prec(precision),

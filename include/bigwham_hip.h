@@ -172,6 +172,11 @@ int bw_hip_set_stacking(int stack_rows);
  * 0 keeps the current value).  Chosen by bigwham_dev_amd.autotune. */
 int bw_hip_set_unroll(bw_hip_handle* hd, int unroll_fp64, int unroll_fp32);
 
+/* The same for one pass (1: t = B x, 2: y = F x + A t) of the precisions in
+ * precision_mask: small pass-1 slabs and long pass-2 slabs can prefer
+ * different depths.  The autotuner picks both from one timing run. */
+int bw_hip_set_unroll_pass(bw_hip_handle* hd, int precision_mask, int pass, int unroll);
+
 /* Schedule of the MFMA kernels of the batched products, for the precisions
  * in precision_mask.  FP32: group pairs per step and whether the next step's
  * loads are issued before the current step's MFMAs (0: 4 pairs; 1: 2 pairs,

@@ -129,14 +129,18 @@ def test_batched_fp32_mfma(gpu, cases, nrhs, original):
             assert rel(Y[:, k], ref) <= TOL32, (name, k)
 
 
-@pytest.mark.parametrize("unroll", [4, 8, 16])
+@pytest.mark.parametrize("unroll", [4, 8, 16, (4, 16), (16, 8)])
 def test_unroll_depths(gpu, cases, unroll):
-    """Every unroll depth the autotuner may pick keeps parity."""
+    """Every unroll depth the autotuner may pick, the same in both passes or
+    one per pass, keeps parity."""
     import torch
     h = cases["3DR0"]
     print(f"unroll {unroll} n={h.size}", flush=True)
     dev = HipHmat(h, precisions=("fp64", "fp32"))
-    dev.set_unroll(fp64=unroll, fp32=unroll)
+    u1, u2 = unroll if isinstance(unroll, tuple) else (unroll, unroll)
+    for p in ("fp64", "fp32"):
+        dev.set_unroll_pass(p, 1, u1)
+        dev.set_unroll_pass(p, 2, u2)
     x = np.random.default_rng(11).uniform(-1, 1, h.size)
     y_ref = O.matvec_original(h.flat(), x)
     xd = torch.from_numpy(x).to(gpu)
@@ -150,7 +154,7 @@ def test_autotune_unroll(gpu, cases, tmp_path, monkeypatch):
     dev = HipHmat(cases["3DR0"], precisions=("fp64", "fp32"))
     r = autotune.tune_unroll(dev, reps=2)
     assert set(r.unroll) == {"fp64", "fp32"}
-    assert all(u in autotune.UNROLLS for u in r.unroll.values())
+    assert all(len(u) == 2 and set(u) <= set(autotune.UNROLLS) for u in r.unroll.values())
     r2 = autotune.tune_unroll(dev, reps=2)  # from the cache
     assert r2.unroll == r.unroll and not r2.ms
 

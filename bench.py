@@ -51,7 +51,8 @@ CONFIGS = {
 # HBM bytes per launch of the slab kernels, measured with rocprofv3 PMC counters
 # (FETCH_SIZE and WRITE_SIZE, scaled by a copy of known size) in separate runs;
 # the counters cannot be read in the timed run itself.
-TRAFFIC_FILES = {"c3": "profiles/hbm_traffic_3dt0_600k_dof.md"}
+TRAFFIC_FILES = {"c2": "profiles/hbm_traffic_3dr0_120k_dof.md",
+                 "c3": "profiles/hbm_traffic_3dt0_600k_dof.md"}
 
 
 def load_traffic(config):

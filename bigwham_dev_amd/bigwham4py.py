@@ -6,7 +6,7 @@ float64, matvec in the original DoF ordering), same helper methods for the
 matvec path: getCompression, getPermutation, getCollocationPoints,
 getSpatialDimension, H_diag, H_jacobi_prec, H_ILU_prec, _getFullBlocks,
 get_diagonal, getMatvecTime, get_max_error_ACA, _getPattern,
-write_hmatrix, get_element_normals, get_rotation_matrix,
+write_hmatrix, plotPattern, get_element_normals, get_rotation_matrix,
 convert_to_global / convert_to_local, and sub-matrix selection by indexing
 (M[np.ix_(rows, cols)]).  Construction stays on the host (libbwhost);
 the matvec runs on the GPU (libbwhip).  `useCuda` is accepted for signature
@@ -163,6 +163,35 @@ class BEMatrix(LinearOperator):
                           (fp[3] - fp[1]) * (fp[4] - fp[2])], 1),
                 np.stack([lp[1], lp[2], lp[3], lp[4], np.ones_like(lp[1]), lp[5]], 1)]
         return np.concatenate(rows, 0)
+
+    def plotPattern(self, plot_index: bool = False):
+        """Figure of the block pattern (point indices, row 0 at the top):
+        full-rank blocks in one colour, low-rank blocks in another, optionally
+        numbered in pattern order within each kind.  Returns the figure."""
+        import matplotlib.pyplot as plt
+        from matplotlib.collections import PatchCollection
+        from matplotlib.patches import Rectangle
+
+        pat = self._getPattern()
+        n = int(pat[:, 2].max())
+        fig, ax = plt.subplots()
+        rects, kinds, seen = [], [], [0, 0]
+        for i0, j0, i1, j1, flag, _ in pat:
+            rects.append(Rectangle((j0, n - i1), j1 - j0, i1 - i0))
+            kinds.append(flag)
+            if plot_index:
+                ax.text((j0 + j1) / 2, n - (i0 + i1) / 2, str(seen[flag]), ha="center",
+                        va="center", fontsize=7)
+            seen[flag] += 1
+        coll = PatchCollection(rects, cmap="coolwarm", edgecolor="black", linewidth=0.3,
+                               alpha=0.5)
+        coll.set_array(np.asarray(kinds, dtype=float))
+        ax.add_collection(coll)
+        ax.set_xlim(0, n)
+        ax.set_ylim(0, n)
+        ax.set_aspect("equal")
+        ax.set_title(f"{seen[0]} full-rank, {seen[1]} low-rank blocks")
+        return fig
 
     def _getFullBlocks(self, original_order: bool = True, keep_ratio: float = 1.0) -> csc_matrix:
         """Entries of the full-rank blocks as a sparse matrix

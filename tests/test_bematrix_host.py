@@ -39,3 +39,15 @@ def test_round_trip_3d():
         nrm = M.get_element_normals().reshape(-1, 3)
         assert np.allclose(xl.reshape(-1, 3)[:, 2],
                            np.einsum("ei,ei->e", nrm, x.reshape(-1, 3)), atol=1e-14)
+
+
+def test_plot_pattern():
+    """One patch per block of the pattern, inside the square of the points."""
+    import matplotlib
+    matplotlib.use("Agg")
+    M = _pattern_only("3DR0-H", rect_grid(8))
+    fig = M.plotPattern(plot_index=True)
+    pat = M._getPattern()
+    coll = fig.axes[0].collections[0]
+    assert len(coll.get_paths()) == pat.shape[0]
+    assert fig.axes[0].get_xlim() == (0, pat[:, 2].max())

@@ -71,15 +71,17 @@ struct Vec<float> {
 // (workgroup b runs on XCD b % 8).  Each XCD gets a contiguous range of work
 // items of about equal bytes (neighbouring row clusters share x entries in
 // that XCD's L2, and no XCD is left with the tail); its workgroups take the
-// range four WIs at a time.  Returns the WI of this wave, or -1.
+// range WAVES WIs at a time (WAVES = waves per workgroup).  Returns the WI of
+// this wave, or -1.
 struct XcdSplit {
   int32_t s[bwl::kXcds + 1];
 };
 
+template <int WAVES = 4>
 __device__ __forceinline__ int xcd_work_item(const XcdSplit& xs) {
   const int x = blockIdx.x % bwl::kXcds;
   const int k = blockIdx.x / bwl::kXcds;
-  const int w = xs.s[x] + 4 * k + static_cast<int>(threadIdx.x >> 6);
+  const int w = xs.s[x] + WAVES * k + static_cast<int>(threadIdx.x >> 6);
   return w < xs.s[x + 1] ? w : -1;
 }
 
@@ -104,12 +106,14 @@ __device__ __forceinline__ int64_t out_index(int64_t idx, const int64_t* perm, i
   return dof * perm[p] + (idx - p * dof);
 }
 
-// One wavefront per work item, four per workgroup.
+// One wavefront per work item and per workgroup: work items differ in size
+// (small pass-1 slabs next to long ones), and a workgroup of several waves
+// would hold its slot until its longest wave ends.
 // T: factor / accumulation type; OutT: output type (T for pass 1, double
 // for pass 2); U: unroll depth (independent 16-byte loads in flight per lane);
 // PASS only names the two launches apart in profiles.
 template <typename T, typename OutT, int U, int PASS>
-__global__ __launch_bounds__(256) void k_slab(const bwl::WorkItem* __restrict__ wis, XcdSplit xs,
+__global__ __launch_bounds__(64) void k_slab(const bwl::WorkItem* __restrict__ wis, XcdSplit xs,
                                               const int32_t* __restrict__ gidx,
                                               const T* __restrict__ data,
                                               const T* __restrict__ xin,
@@ -119,7 +123,7 @@ __global__ __launch_bounds__(256) void k_slab(const bwl::WorkItem* __restrict__ 
   typedef typename Vec<T>::type VT;
   constexpr int V = Vec<T>::V;
   const int lane = threadIdx.x & 63;
-  const int w = xcd_work_item(xs);
+  const int w = xcd_work_item<1>(xs);
   if (w < 0) return;
   const bwl::WorkItem wi = wis[w];
   const int R = wi.rows;
@@ -1044,15 +1048,18 @@ template <typename T, typename OutT, int PASS>
 void launch_slab(int unroll, int blocks, hipStream_t st, const bwl::WorkItem* wis, XcdSplit xs,
                  const int32_t* gidx, const T* data, const T* xin, OutT* out, T* part,
                  const int64_t* perm, int dof, const int32_t* rowmap) {
+  // `blocks` counts four-wave workgroups (the layout's XCD split); k_slab
+  // runs one wave per workgroup
+  constexpr int kWavesPerGroup = 4;
   if (unroll == 4)
-    hipLaunchKernelGGL((k_slab<T, OutT, 4, PASS>), dim3(blocks), dim3(256), 0, st, wis, xs, gidx,
-                       data, xin, out, part, perm, dof, rowmap);
+    hipLaunchKernelGGL((k_slab<T, OutT, 4, PASS>), dim3(kWavesPerGroup * blocks), dim3(64), 0,
+                       st, wis, xs, gidx, data, xin, out, part, perm, dof, rowmap);
   else if (unroll == 16)
-    hipLaunchKernelGGL((k_slab<T, OutT, 16, PASS>), dim3(blocks), dim3(256), 0, st, wis, xs, gidx,
-                       data, xin, out, part, perm, dof, rowmap);
+    hipLaunchKernelGGL((k_slab<T, OutT, 16, PASS>), dim3(kWavesPerGroup * blocks), dim3(64), 0,
+                       st, wis, xs, gidx, data, xin, out, part, perm, dof, rowmap);
   else
-    hipLaunchKernelGGL((k_slab<T, OutT, 8, PASS>), dim3(blocks), dim3(256), 0, st, wis, xs, gidx,
-                       data, xin, out, part, perm, dof, rowmap);
+    hipLaunchKernelGGL((k_slab<T, OutT, 8, PASS>), dim3(kWavesPerGroup * blocks), dim3(64), 0,
+                       st, wis, xs, gidx, data, xin, out, part, perm, dof, rowmap);
 }
 
 template <typename T>

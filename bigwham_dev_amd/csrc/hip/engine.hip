@@ -472,7 +472,7 @@ __device__ __forceinline__ void mfma_stream_deep(const flt4* __restrict__ base,
 }
 
 template <int PASS, int KU, bool PIPE, int D = 0>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_slab_mfma(
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_slab_mfma(
     const bwl::WorkItem* __restrict__ wis, XcdSplit xs, const int32_t* __restrict__ gidx,
     const float* __restrict__ data, const float* __restrict__ XB,
     float* __restrict__ tout,              // pass 1
@@ -480,7 +480,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     int nrhs, float* __restrict__ part, const int64_t* __restrict__ perm, int dof,
     const int32_t* __restrict__ rowmap) {
   const int lane = threadIdx.x & 63;
-  const int w = xcd_work_item(xs);
+  const int w = xcd_work_item<1>(xs);
   if (w < 0) return;
   const bwl::WorkItem wi = wis[w];
   const int R = wi.rows;
@@ -539,35 +539,35 @@ void launch_mfma(int schedule, int blocks, hipStream_t st, const bwl::WorkItem* 
                  const int32_t* rowmap) {
   switch (schedule) {
     case 1:
-      hipLaunchKernelGGL((k_slab_mfma<PASS, 2, true>), dim3(blocks), dim3(256), 0, st, wis, xs,
+      hipLaunchKernelGGL((k_slab_mfma<PASS, 2, true>), dim3(4 * blocks), dim3(64), 0, st, wis, xs,
                          gidx, data, XB, tout, Y, yr, yc, nrhs, part, perm, dof, rowmap);
       break;
     case 2:
-      hipLaunchKernelGGL((k_slab_mfma<PASS, 4, true>), dim3(blocks), dim3(256), 0, st, wis, xs,
+      hipLaunchKernelGGL((k_slab_mfma<PASS, 4, true>), dim3(4 * blocks), dim3(64), 0, st, wis, xs,
                          gidx, data, XB, tout, Y, yr, yc, nrhs, part, perm, dof, rowmap);
       break;
     case 3:
-      hipLaunchKernelGGL((k_slab_mfma<PASS, 8, false>), dim3(blocks), dim3(256), 0, st, wis, xs,
+      hipLaunchKernelGGL((k_slab_mfma<PASS, 8, false>), dim3(4 * blocks), dim3(64), 0, st, wis, xs,
                          gidx, data, XB, tout, Y, yr, yc, nrhs, part, perm, dof, rowmap);
       break;
     case 4:
-      hipLaunchKernelGGL((k_slab_mfma<PASS, 2, true, 3>), dim3(blocks), dim3(256), 0, st, wis, xs,
+      hipLaunchKernelGGL((k_slab_mfma<PASS, 2, true, 3>), dim3(4 * blocks), dim3(64), 0, st, wis, xs,
                          gidx, data, XB, tout, Y, yr, yc, nrhs, part, perm, dof, rowmap);
       break;
     case 5:
-      hipLaunchKernelGGL((k_slab_mfma<PASS, 2, true, 4>), dim3(blocks), dim3(256), 0, st, wis, xs,
+      hipLaunchKernelGGL((k_slab_mfma<PASS, 2, true, 4>), dim3(4 * blocks), dim3(64), 0, st, wis, xs,
                          gidx, data, XB, tout, Y, yr, yc, nrhs, part, perm, dof, rowmap);
       break;
     case 6:
-      hipLaunchKernelGGL((k_slab_mfma<PASS, 1, true, 6>), dim3(blocks), dim3(256), 0, st, wis, xs,
+      hipLaunchKernelGGL((k_slab_mfma<PASS, 1, true, 6>), dim3(4 * blocks), dim3(64), 0, st, wis, xs,
                          gidx, data, XB, tout, Y, yr, yc, nrhs, part, perm, dof, rowmap);
       break;
     case 7:
-      hipLaunchKernelGGL((k_slab_mfma<PASS, 1, true, 8>), dim3(blocks), dim3(256), 0, st, wis, xs,
+      hipLaunchKernelGGL((k_slab_mfma<PASS, 1, true, 8>), dim3(4 * blocks), dim3(64), 0, st, wis, xs,
                          gidx, data, XB, tout, Y, yr, yc, nrhs, part, perm, dof, rowmap);
       break;
     default:
-      hipLaunchKernelGGL((k_slab_mfma<PASS, 4, false>), dim3(blocks), dim3(256), 0, st, wis, xs,
+      hipLaunchKernelGGL((k_slab_mfma<PASS, 4, false>), dim3(4 * blocks), dim3(64), 0, st, wis, xs,
                          gidx, data, XB, tout, Y, yr, yc, nrhs, part, perm, dof, rowmap);
   }
 }
@@ -634,7 +634,7 @@ typedef double dblx4 __attribute__((ext_vector_type(4)));
 // then their MFMAs.  Both issue the same MFMAs in the same order per
 // accumulator, so the results agree bit for bit.
 template <int PASS, int D>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(D > 4 ? 2 : (D > 0 ? 3 : 4)))) void k_slab_mfma64(
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(D > 4 ? 2 : (D > 0 ? 3 : 4)))) void k_slab_mfma64(
     const bwl::WorkItem* __restrict__ wis, XcdSplit xs, const int32_t* __restrict__ gidx,
     const double* __restrict__ data, const double* __restrict__ XB,
     double* __restrict__ tout,             // pass 1
@@ -642,7 +642,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(D > 4 ? 2 :
     int nrhs, double* __restrict__ part, const int64_t* __restrict__ perm, int dof,
     const int32_t* __restrict__ rowmap) {
   const int lane = threadIdx.x & 63;
-  const int w = xcd_work_item(xs);
+  const int w = xcd_work_item<1>(xs);
   if (w < 0) return;
   const bwl::WorkItem wi = wis[w];
   const int R = wi.rows;
@@ -758,7 +758,7 @@ void launch_mfma64(int schedule, int blocks, hipStream_t st, const bwl::WorkItem
                    double* Y, int64_t yr, int64_t yc, int nrhs, double* part,
                    const int64_t* perm, int dof, const int32_t* rowmap) {
 #define BW_MFMA64(D)                                                                        \
-  hipLaunchKernelGGL((k_slab_mfma64<PASS, D>), dim3(blocks), dim3(256), 0, st, wis, xs, gidx, \
+  hipLaunchKernelGGL((k_slab_mfma64<PASS, D>), dim3(4 * blocks), dim3(64), 0, st, wis, xs, gidx, \
                      data, XB, tout, Y, yr, yc, nrhs, part, perm, dof, rowmap)
   switch (schedule) {
     case 4: BW_MFMA64(3); break;

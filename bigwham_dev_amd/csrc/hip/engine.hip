@@ -102,8 +102,10 @@ __global__ __launch_bounds__(256) void k_gather(const double* __restrict__ x,
 template <typename OutT>
 __device__ __forceinline__ int64_t out_index(int64_t idx, const int64_t* perm, int dof) {
   if (!perm) return idx;
-  const int64_t p = idx / dof;
-  return dof * perm[p] + (idx - p * dof);
+  // row indices fit in 32 bits (checked when the layout is built)
+  const int i = static_cast<int>(idx);
+  const int p = i / dof;
+  return dof * perm[p] + (i - p * dof);
 }
 
 // One wavefront per work item and per workgroup: work items differ in size
@@ -332,6 +334,43 @@ __global__ __launch_bounds__(256) void k_gather_b(const double* __restrict__ X, 
 // loads valid data; a group past the end gets B = 0 (its A rows then add
 // nothing), and rows past R are never stored.  The mask is applied at the
 // MFMA, not at the load, so no VALU op waits on a load of the next step.
+// Output of the batched kernels: pass 1 writes t (row through rowmap, kRhs
+// values per row), pass 2 writes Y (row through the point permutation, row
+// stride yr, column stride yc).
+// batched_rows maps N output rows of a work item (indices already clamped
+// into range) to the rows they are stored at, with straight-line code per
+// case, so that all index loads go out before the first one is waited for:
+// pass 1 through rowmap, pass 2 through the point permutation.  Row indices
+// and permutation entries are below 2^31 (checked when the layout is built),
+// so the low word of each permutation entry is the whole value.
+template <int PASS, int N>
+__device__ __forceinline__ void batched_rows(const int (&idx)[N], const int32_t* rowmap,
+                                             const int64_t* perm, int dof, int (&out)[N]) {
+  if (PASS == 1) {
+#pragma unroll
+    for (int j = 0; j < N; ++j) out[j] = rowmap[idx[j]];
+  } else if (perm) {
+    const int32_t* plo = reinterpret_cast<const int32_t*>(perm);
+    int pv[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) pv[j] = plo[2 * (idx[j] / dof)];
+#pragma unroll
+    for (int j = 0; j < N; ++j) out[j] = dof * pv[j] + idx[j] % dof;
+  } else {
+#pragma unroll
+    for (int j = 0; j < N; ++j) out[j] = idx[j];
+  }
+}
+
+template <int PASS, typename T>
+__device__ __forceinline__ void batched_put(T* tout, double* Y, int64_t yr, int64_t yc, int row,
+                                            int n, T v) {
+  if (PASS == 1)
+    tout[static_cast<int64_t>(row) * kRhs + n] = v;
+  else
+    Y[row * yr + n * yc] = static_cast<double>(v);
+}
+
 template <int KU, bool PIPE, bool TWO>
 __device__ __forceinline__ void mfma_stream(const flt4* __restrict__ base,
                                             const int32_t* __restrict__ gi,
@@ -506,19 +545,25 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
   }
   const int n = lane & 31;
   auto row_of = [&](int j) { return 8 * (j >> 2) + 4 * half + (j & 3); };
-  auto store = [&](int row, float v, int64_t out_row) {
-    if (row >= R || n >= nrhs) return;
-    if (PASS == 1) {
-      tout[static_cast<int64_t>(rowmap[out_row + row]) * kRhs + n] = v;
-    } else {
-      Y[out_index<double>(out_row + row, perm, dof) * yr + n * yc] = static_cast<double>(v);
-    }
-  };
   if (wi.part_off < 0) {
+    // 8 output rows at a time: their index loads (clamped rows, all in
+    // bounds, none behind a branch) go out together, then the stores, instead
+    // of one round trip per stored value
+    auto put8 = [&](int r0, int j0, const floatx16& acc) {
+      int idx[8], orow[8];
 #pragma unroll
-    for (int j = 0; j < 16; ++j) {
-      store(row_of(j), acc0[j], wi.out_row);
-      if (two) store(32 + row_of(j), acc1[j], wi.out_row);
+      for (int j = 0; j < 8; ++j) idx[j] = wi.out_row + min(r0 + row_of(j0 + j), R - 1);
+      batched_rows<PASS>(idx, rowmap, perm, dof, orow);
+      if (n >= nrhs) return;
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        if (r0 + row_of(j0 + j) < R) batched_put<PASS>(tout, Y, yr, yc, orow[j], n, acc[j0 + j]);
+    };
+    put8(0, 0, acc0);
+    put8(0, 8, acc0);
+    if (two) {
+      put8(32, 0, acc1);
+      put8(32, 8, acc1);
     }
     return;
   }
@@ -728,27 +773,42 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(D > 4 ? 2 : 
         }
   }
   }
-  double* pp = wi.part_off >= 0 ? part + static_cast<int64_t>(wi.part_off) * kRhs : nullptr;
+  if (wi.part_off >= 0) {
+    double* pp = part + static_cast<int64_t>(wi.part_off) * kRhs;
 #pragma unroll
-  for (int t = 0; t < 4; ++t) {
-    if (t >= ntile) break;
+    for (int t = 0; t < 4; ++t) {
+      if (t >= ntile) break;
 #pragma unroll
-    for (int c = 0; c < 2; ++c)
+      for (int c = 0; c < 2; ++c)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = 16 * t + k + 4 * r;
-        const int n = 16 * c + i16;
-        if (row >= R) continue;
-        const double val = acc[t][c][r];
-        if (pp) {
-          pp[row * kRhs + n] = val;  // partial sums, added in order by k_reduce_b
-        } else if (n < nrhs) {
-          if (PASS == 1)
-            tout[static_cast<int64_t>(rowmap[wi.out_row + row]) * kRhs + n] = val;
-          else
-            Y[out_index<double>(static_cast<int64_t>(wi.out_row) + row, perm, dof) * yr + n * yc] = val;
+        for (int r = 0; r < 4; ++r) {
+          const int row = 16 * t + k + 4 * r;
+          if (row < R) pp[row * kRhs + 16 * c + i16] = acc[t][c][r];  // added in order by k_reduce_b
         }
-      }
+    }
+    return;
+  }
+  // two tiles (8 of this lane's rows) at a time: their output rows first
+  // (clamped, in bounds, no branch), so the index loads go out together, then
+  // the stores
+#pragma unroll
+  for (int t0 = 0; t0 < 4; t0 += 2) {
+    if (t0 >= ntile) break;
+    int idx[8], orow[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) idx[j] = wi.out_row + min(16 * (t0 + (j >> 2)) + k + 4 * (j & 3), R - 1);
+    batched_rows<PASS>(idx, rowmap, perm, dof, orow);
+#pragma unroll
+    for (int t = t0; t < t0 + 2; ++t)
+#pragma unroll
+      for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = 16 * t + k + 4 * r;
+          const int n = 16 * c + i16;
+          if (row < R && n < nrhs)
+            batched_put<PASS>(tout, Y, yr, yc, orow[4 * (t - t0) + r], n, acc[t][c][r]);
+        }
   }
 }
 

@@ -493,6 +493,10 @@ int check_layout_limits(const Layout& L, std::string* msg) {
     *msg = "input and rank vectors too long for 32-bit offsets";
     return BW_ESHAPE;
   }
+  if (L.view.n_row_pts * L.view.dof >= lim) {
+    *msg = "too many rows for 32-bit row indices";
+    return BW_ESHAPE;
+  }
   for (const Program* P : {&L.pass1, &L.pass2}) {
     if (static_cast<int64_t>(P->gidx.size()) >= lim || P->part_elems >= lim ||
         static_cast<int64_t>(P->rowmap.size()) >= lim ||

@@ -921,12 +921,16 @@ int upload_pass(const bwl::Layout& L, const bwl::Program& P, DevPass* dp, hipStr
   HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&stage[0]), chunk_elems * sizeof(T)));
   hipError_t e2 = hipHostMalloc(reinterpret_cast<void**>(&stage[1]), chunk_elems * sizeof(T));
   if (e2 != hipSuccess) {
-    hipHostFree(stage[0]);
+    (void)hipHostFree(stage[0]);
     return fail(BW_ENOMEM, "hipHostMalloc staging");
   }
-  hipEvent_t done[2];
-  hipEventCreate(&done[0]);
-  hipEventCreate(&done[1]);
+  hipEvent_t done[2] = {nullptr, nullptr};
+  if (hipEventCreate(&done[0]) != hipSuccess || hipEventCreate(&done[1]) != hipSuccess) {
+    if (done[0]) (void)hipEventDestroy(done[0]);
+    (void)hipHostFree(stage[0]);
+    (void)hipHostFree(stage[1]);
+    return fail(BW_EHIP, "upload: hipEventCreate");
+  }
   const unsigned hc = std::max(1u, std::thread::hardware_concurrency());
   const int nthreads = static_cast<int>(std::min(hc, 32u));
   int buf = 0;
@@ -943,8 +947,13 @@ int upload_pass(const bwl::Layout& L, const bwl::Program& P, DevPass* dp, hipStr
       ++e;
     }
     if (pending[buf]) {
-      hipEventSynchronize(done[buf]);
+      // the copy out of this staging buffer must be complete before refilling it
+      const hipError_t we = hipEventSynchronize(done[buf]);
       pending[buf] = false;
+      if (we != hipSuccess) {
+        status = fail(BW_EHIP, std::string("upload: ") + hipGetErrorString(we));
+        break;
+      }
     }
     T* dst = stage[buf];
     // parallel fill over slabs of this chunk
@@ -967,18 +976,25 @@ int upload_pass(const bwl::Layout& L, const bwl::Program& P, DevPass* dp, hipStr
       status = fail(BW_EHIP, std::string("upload: ") + hipGetErrorString(ce));
       break;
     }
-    hipEventRecord(done[buf], st);
+    const hipError_t re = hipEventRecord(done[buf], st);
+    if (re != hipSuccess) {
+      status = fail(BW_EHIP, std::string("upload: ") + hipGetErrorString(re));
+      break;
+    }
     pending[buf] = true;
     buf ^= 1;
     if (verbose() && nslab >= 10 && (e * 10 / nslab) != (s * 10 / nslab))
       std::fprintf(stderr, "[bwhip] upload %d / %d slabs\n", e, nslab);
     s = e;
   }
-  hipStreamSynchronize(st);
-  hipEventDestroy(done[0]);
-  hipEventDestroy(done[1]);
-  hipHostFree(stage[0]);
-  hipHostFree(stage[1]);
+  // every copy has ended (or failed) before the staging buffers are freed
+  const hipError_t se = hipStreamSynchronize(st);
+  if (se != hipSuccess && status == BW_OK)
+    status = fail(BW_EHIP, std::string("upload: ") + hipGetErrorString(se));
+  (void)hipEventDestroy(done[0]);
+  (void)hipEventDestroy(done[1]);
+  (void)hipHostFree(stage[0]);
+  (void)hipHostFree(stage[1]);
   return status;
 }
 
@@ -1130,7 +1146,7 @@ int launch_matvec(bw_hip_handle* hd, DevOp& op, const double* xd, double* yd, in
   T* xt = static_cast<T*>(op.xt);
   double* const y_user = yd;
   if (sd) yd = hd->sel.y_full;  // the passes write the full permuted y
-  if (hd->timing) hipEventRecord(hd->ev[0], st);
+  if (hd->timing) (void)hipEventRecord(hd->ev[0], st);
   if (sd) {
     if (sd->n_pairs > 0) {
       const int blocks = static_cast<int>((sd->n_pairs * dof + 255) / 256);
@@ -1143,25 +1159,25 @@ int launch_matvec(bw_hip_handle* hd, DevOp& op, const double* xd, double* yd, in
     hipLaunchKernelGGL(k_gather<T>, dim3(blocks), dim3(256), 0, st, xd, xt, n,
                        orig ? op.perm_in : nullptr, dof);
   }
-  if (hd->timing) hipEventRecord(hd->ev[1], st);
+  if (hd->timing) (void)hipEventRecord(hd->ev[1], st);
   if (op.p1.nwi > 0) {
     launch_slab<T, T, 1>(op.unroll[0], op.p1.blocks, st, op.p1.wis, op.p1.xs, op.p1.gidx,
                          static_cast<const T*>(op.p1.data), xt, xt, static_cast<T*>(op.p1.part),
                          nullptr, dof, op.p1.rowmap);
   }
-  if (hd->timing) hipEventRecord(hd->ev[2], st);
+  if (hd->timing) (void)hipEventRecord(hd->ev[2], st);
   if (op.p1.nred > 0)
     hipLaunchKernelGGL((k_reduce<T, T>), dim3(op.p1.nred), dim3(64), 0, st, op.p1.reds,
                        op.p1.nred, static_cast<const T*>(op.p1.part), xt, nullptr, dof,
                        op.p1.rowmap);
-  if (hd->timing) hipEventRecord(hd->ev[3], st);
+  if (hd->timing) (void)hipEventRecord(hd->ev[3], st);
   if (op.p2.nwi > 0) {
     launch_slab<T, double, 2>(op.unroll[1], op.p2.blocks, st, op.p2.wis, op.p2.xs, op.p2.gidx,
                               static_cast<const T*>(op.p2.data), xt, yd,
                               static_cast<T*>(op.p2.part), orig ? op.perm_out : nullptr, dof,
                               nullptr);
   }
-  if (hd->timing) hipEventRecord(hd->ev[4], st);
+  if (hd->timing) (void)hipEventRecord(hd->ev[4], st);
   if (op.p2.nred > 0)
     hipLaunchKernelGGL((k_reduce<T, double>), dim3(op.p2.nred), dim3(64), 0, st, op.p2.reds,
                        op.p2.nred, static_cast<const T*>(op.p2.part), yd,
@@ -1171,7 +1187,7 @@ int launch_matvec(bw_hip_handle* hd, DevOp& op, const double* xd, double* yd, in
     hipLaunchKernelGGL(k_gather_sel, dim3(blocks), dim3(256), 0, st, yd, y_user, sd->out_dst,
                        sd->n_out, dof);
   }
-  if (hd->timing) hipEventRecord(hd->ev[5], st);
+  if (hd->timing) (void)hipEventRecord(hd->ev[5], st);
   HIPCHK(hipGetLastError());
   return BW_OK;
 }
@@ -1263,12 +1279,12 @@ int bw_hip_set_stacking(int stack_rows) {
 
 void bw_hip_destroy(bw_hip_handle* hd) {
   if (!hd) return;
-  hipSetDevice(hd->device);
-  hipDeviceSynchronize();
-  for (void* p : hd->allocs) hipFree(p);
+  (void)hipSetDevice(hd->device);
+  (void)hipDeviceSynchronize();
+  for (void* p : hd->allocs) (void)hipFree(p);
   for (auto& set : hd->evs)
     for (auto& e : set)
-      if (e) hipEventDestroy(e);
+      if (e) (void)hipEventDestroy(e);
   delete hd;
 }
 
@@ -1330,19 +1346,19 @@ int create_impl(const bw_flat_hmat* h, long double expected_area, int precision_
     if (precision_mask & BW_PREC_FP64) {
       rc = build_op<double>(hd, h, adj != 0, st);
       if (rc) {
-        hipStreamDestroy(st);
+        (void)hipStreamDestroy(st);
         return cleanup(rc);
       }
     }
     if (precision_mask & BW_PREC_FP32) {
       rc = build_op<float>(hd, h, adj != 0, st);
       if (rc) {
-        hipStreamDestroy(st);
+        (void)hipStreamDestroy(st);
         return cleanup(rc);
       }
     }
   }
-  hipStreamDestroy(st);
+  (void)hipStreamDestroy(st);
   // nbOfEntries (hmat.cc:299-312)
   int64_t E = 0;
   for (int64_t k = 0; k < h->n_fr; ++k) {
@@ -1560,9 +1576,9 @@ int bw_hip_graph_create(bw_hip_handle* hd, const double* x, double* y, int order
     rc = fail(BW_EHIP, std::string("graph_create: ") + hipGetErrorString(e));
   }
   hd->timing = timing;
-  hipStreamDestroy(cs);
+  (void)hipStreamDestroy(cs);
   if (rc) {
-    if (graph) hipGraphDestroy(graph);
+    if (graph) (void)hipGraphDestroy(graph);
     return rc;
   }
   auto* g = new bw_hip_graph();
@@ -1570,7 +1586,7 @@ int bw_hip_graph_create(bw_hip_handle* hd, const double* x, double* y, int order
   g->graph = graph;
   e = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
   if (e != hipSuccess) {
-    hipGraphDestroy(graph);
+    (void)hipGraphDestroy(graph);
     delete g;
     return fail(BW_EHIP, std::string("graph_create: instantiate: ") + hipGetErrorString(e));
   }
@@ -1587,10 +1603,10 @@ int bw_hip_graph_launch(bw_hip_graph* g, void* stream) {
 
 void bw_hip_graph_destroy(bw_hip_graph* g) {
   if (!g) return;
-  hipSetDevice(g->device);
-  hipDeviceSynchronize();
-  if (g->exec) hipGraphExecDestroy(g->exec);
-  if (g->graph) hipGraphDestroy(g->graph);
+  (void)hipSetDevice(g->device);
+  (void)hipDeviceSynchronize();
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
   delete g;
 }
 

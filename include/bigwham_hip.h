@@ -51,8 +51,8 @@ typedef struct bw_hip_handle bw_hip_handle;
 /* Builds the device layout and uploads it (replaces the HmatCuda constructor
  * and copyToDevice, hmat_cuda.cc:98-1183).  The host arrays of `h` may be
  * freed after return.  device: HIP device ordinal (-1: current).
- * BW_ESHAPE if the matrix does not fit the layout's 32-bit indices (2^31
- * rows, input plus rank entries, or column groups). */
+ * BW_ESHAPE if the matrix does not fit the layout's 32-bit indices (2^30
+ * rows or columns, input plus rank entries, or column groups). */
 int bw_hip_create(const bw_flat_hmat* h, int precision_mask, int flags, int device,
                   bw_hip_handle** out);
 

@@ -1,0 +1,67 @@
+"""Rectangular operators on the device: receiver and source point sets of
+different sizes, as the reference's Hmat between two meshes, with random
+blocks of ranks 0 to 70 (synthetic_hmat.py).  H x, H^T x, FP32 factors and the
+batched MFMA path in both numberings, against the oracle (restatement of
+Hmat::matvec / matvecOriginal, src/hmat/hmatrix/hmat.cc:472-715) and the
+numpy assembly of the same blocks."""
+import numpy as np
+import pytest
+
+import oracle as O
+from bigwham_dev_amd import HipHmat
+from synthetic_hmat import SyntheticHmat
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(150, 90, 3), (70, 260, 2)]
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.fixture(scope="module", params=SHAPES, ids=lambda s: f"{s[0]}x{s[1]}d{s[2]}")
+def rect(request, gpu):
+    S = SyntheticHmat(*request.param, seed=sum(request.param))
+    print(f"rectangular {request.param}", flush=True)
+    dev = HipHmat(S, precisions=("fp64", "fp32"), adjoint=True, device=0)
+    assert (dev.n_rows, dev.n_cols) == (S.dof * S.n_row_pts, S.dof * S.n_col_pts)
+    yield S, dev
+    dev.close()
+
+
+@pytest.mark.parametrize("original", [False, True])
+def test_rect_matvec_and_adjoint(rect, gpu, original):
+    import torch
+    S, dev = rect
+    f = S.flat()
+    rng = np.random.default_rng(1)
+    x = rng.uniform(-1, 1, dev.n_cols)
+    ref = (O.matvec_original if original else O.matvec)(f, x)
+    assert rel(ref, S.dense(original) @ x) < 1e-13
+    xd = torch.from_numpy(x).to(gpu)
+    assert rel(dev.matvec(xd, original=original).cpu().numpy(), ref) <= 1e-12
+    assert rel(dev.matvec(xd, original=original, precision="fp32").cpu().numpy(), ref) <= 1e-5
+    u = rng.uniform(-1, 1, dev.n_rows)
+    reft = (O.rmatvec_original if original else O.rmatvec)(f, u)
+    assert rel(reft, S.dense(original).T @ u) < 1e-13
+    ud = torch.from_numpy(u).to(gpu)
+    assert rel(dev.rmatvec(ud, original=original).cpu().numpy(), reft) <= 1e-12
+    assert rel(dev.rmatvec(ud, original=original, precision="fp32").cpu().numpy(), reft) <= 1e-5
+
+
+@pytest.mark.parametrize("nrhs", [5, 32])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("adjoint", [False, True])
+def test_rect_batched(rect, gpu, nrhs, precision, adjoint):
+    import torch
+    S, dev = rect
+    H = S.dense(True)
+    H = H.T if adjoint else H
+    X = np.random.default_rng(2).uniform(-1, 1, (H.shape[1], nrhs))
+    ref = H @ X
+    Y = dev.matvec_batched(torch.from_numpy(X).to(gpu), precision=precision,
+                           adjoint=adjoint).cpu().numpy()
+    tol = 1e-12 if precision == "fp64" else 1e-5
+    for j in range(nrhs):
+        assert rel(Y[:, j], ref[:, j]) <= tol

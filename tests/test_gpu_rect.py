@@ -65,3 +65,33 @@ def test_rect_batched(rect, gpu, nrhs, precision, adjoint):
     tol = 1e-12 if precision == "fp64" else 1e-5
     for j in range(nrhs):
         assert rel(Y[:, j], ref[:, j]) <= tol
+
+
+@pytest.mark.parametrize("original", [False, True])
+def test_rect_selection(rect, gpu, original):
+    """H[rows, cols] of a rectangular operator (bw_hip_create_selection):
+    rows index the receiver points, columns the source points."""
+    import torch
+    S, _ = rect
+    d = S.dof
+    rng = np.random.default_rng(3)
+    rows = rng.choice(S.n_row_pts, S.n_row_pts // 3, replace=False)
+    cols = rng.choice(S.n_col_pts, S.n_col_pts // 2, replace=False)
+    dev = HipHmat.selection(S, rows, cols, original=original, precisions=("fp64", "fp32"),
+                            adjoint=True, device=0)
+    assert (dev.n_rows, dev.n_cols) == (d * rows.size, d * cols.size)
+    ri = (d * rows[:, None] + np.arange(d)).ravel()
+    ci = (d * cols[:, None] + np.arange(d)).ravel()
+    Hs = S.dense(original)[np.ix_(ri, ci)]
+    x = rng.uniform(-1, 1, d * cols.size)
+    ref = O.selection_matvec(S.flat(), rows, cols, x, original=original)
+    assert rel(ref, Hs @ x) < 1e-13
+    xd = torch.from_numpy(x).to(gpu)
+    assert rel(dev.matvec(xd, original=original).cpu().numpy(), ref) <= 1e-12
+    assert rel(dev.matvec(xd, original=original, precision="fp32").cpu().numpy(), ref) <= 1e-5
+    u = rng.uniform(-1, 1, d * rows.size)
+    reft = O.selection_matvec(S.flat(), rows, cols, u, original=original, adjoint=True)
+    assert rel(reft, Hs.T @ u) < 1e-13
+    assert rel(dev.rmatvec(torch.from_numpy(u).to(gpu), original=original).cpu().numpy(),
+               reft) <= 1e-12
+    dev.close()

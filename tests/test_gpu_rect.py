@@ -95,3 +95,26 @@ def test_rect_selection(rect, gpu, original):
     assert rel(dev.rmatvec(torch.from_numpy(u).to(gpu), original=original).cpu().numpy(),
                reft) <= 1e-12
     dev.close()
+
+
+def test_rect_composite(gpu):
+    """Rectangular operators of different shapes as one block-diagonal
+    operator (bw_hip_create_multi): each output slice equals its own
+    operator's product."""
+    import torch
+    mats = [SyntheticHmat(150, 90, 3, seed=21), SyntheticHmat(40, 130, 3, seed=22),
+            SyntheticHmat(95, 95, 3, seed=23)]
+    dev = HipHmat.many(mats, precisions=("fp64", "fp32"), adjoint=True, device=0)
+    ro, co = dev.row_offsets, dev.col_offsets
+    assert ro[-1] == dev.n_rows and co[-1] == dev.n_cols
+    rng = np.random.default_rng(4)
+    x = rng.uniform(-1, 1, dev.n_cols)
+    u = rng.uniform(-1, 1, dev.n_rows)
+    for original in (False, True):
+        y = dev.matvec(torch.from_numpy(x).to(gpu), original=original).cpu().numpy()
+        yt = dev.rmatvec(torch.from_numpy(u).to(gpu), original=original).cpu().numpy()
+        for k, S in enumerate(mats):
+            H = S.dense(original)
+            assert rel(y[ro[k]:ro[k + 1]], H @ x[co[k]:co[k + 1]]) <= 1e-12
+            assert rel(yt[co[k]:co[k + 1]], H.T @ u[ro[k]:ro[k + 1]]) <= 1e-12
+    dev.close()

@@ -72,3 +72,24 @@ def _rotations(h, n_elts):
     out = np.empty(n_elts * 4)
     N.check_host(N.host().bwh_rotation_matrices(h._h, N.dptr(out)))
     return out.reshape(n_elts, 2, 2)
+
+
+def test_penny_105k_pattern():
+    """The 600k-DoF penny-crack mesh (meshes.penny_delaunay(105000, seed=0)):
+    clusters of more than 60 000 points there have exactly tied split
+    coordinates, where the exchange-sort replay stops after a swap budget
+    (hb_cluster.cpp).  The permutation and the block pattern still equal the
+    compiled reference's (tests/golden/make_c3_pattern_fixture.py)."""
+    import hashlib
+    f = np.load(Path(__file__).resolve().parent / "golden" / "ref_pattern_penny_105k.npz",
+                allow_pickle=False)
+    digest = lambda a: hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+    coor, conn = penny_delaunay(105000, seed=0)
+    h = HostHmat.from_mesh("3DT0-H", coor, conn, [1.0, 0.25])
+    assert digest(h.collocation_points()) == str(f["points_sha256"]), "mesh differs from the fixture's"
+    h.build_pattern(64, 3.0)
+    assert np.array_equal(h.permutation(), f["perm"].astype(np.int64))
+    fr, lr = h.fr_pattern(), h.lr_pattern()
+    assert (fr.shape[1], lr.shape[1]) == (int(f["n_fr"]), int(f["n_lr"]))
+    assert digest(fr[:5].astype(np.int64)) == str(f["fr_sha256"])
+    assert digest(lr[:5].astype(np.int64)) == str(f["lr_sha256"])

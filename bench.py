@@ -418,10 +418,8 @@ def bench_batching(args, cfg):
         f"single fp64 {t_s64 * 1e3:.3f} ms; rel err vs fp64 {err:.2e}")
 
     # 16 uneven square cracks in one operator
-    mrng = np.random.default_rng(7)
     lo, hi = cfg["mesh_elems"]
-    counts = np.exp(mrng.uniform(np.log(lo), np.log(hi), cfg["n_meshes"]))
-    sides = [max(4, int(round(np.sqrt(c)))) for c in counts]
+    sides = meshes.crack_family(cfg["n_meshes"], lo, hi, seed=7)
     t0 = time.perf_counter()
     hosts = [_build_host(cfg["kernel"], *meshes.rect_grid(m), cfg) for m in sides]
     t_mbuild = time.perf_counter() - t0

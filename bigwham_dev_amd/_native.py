@@ -130,6 +130,10 @@ def hip() -> C.CDLL:
                                             C.c_int, C.c_int, C.POINTER(_vp), _i64, _i64]
         lib.bw_hip_graph_create.argtypes = [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int,
                                             C.POINTER(_vp)]
+        lib.bw_hip_graph_create_batched.argtypes = [_vp, C.c_int, C.c_int, _vp, C.c_int64,
+                                                    C.c_int64, _vp, C.c_int64, C.c_int64,
+                                                    C.c_int, C.c_int, C.POINTER(_vp)]
+        lib.bw_hip_stream_release.argtypes = [_vp, _vp]
         lib.bw_hip_graph_launch.argtypes = [_vp, _vp]
         lib.bw_hip_graph_destroy.argtypes = [_vp]
         lib.bw_hip_graph_destroy.restype = None
@@ -182,6 +186,7 @@ HIP_SYMBOLS = ["bw_hip_create", "bw_hip_matvec", "bw_hip_rmatvec", "bw_hip_matve
                "bw_layout_export_rowmap", "bw_layout_export_xcd",
                "bw_layout_last_error", "bw_layout_build_multi", "bw_hip_create_multi",
                "bw_hip_graph_create", "bw_hip_graph_launch", "bw_hip_graph_destroy",
+               "bw_hip_graph_create_batched", "bw_hip_stream_release",
                "bw_flat_compose", "bw_flat_composite_view", "bw_flat_composite_offsets",
                "bw_flat_compose_last_error", "bw_flat_composite_free",
                "bw_hip_create_selection", "bw_flat_full_blocks",

@@ -19,6 +19,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -838,19 +839,21 @@ struct DevPass {
   int32_t* rowmap = nullptr;        // pass 1: t entry of every slab row
   bwl::Reduce* reds = nullptr;      // one per split slab
   void* data = nullptr;
-  void* part = nullptr;
   int nwi = 0, nred = 0;
   XcdSplit xs = {};
   int blocks = 0;  // workgroups of a slab launch
   int64_t data_bytes = 0;
 };
 
+// The read-only device layout of one operator (H or H^T) at one precision.
+// Everything a matvec writes lives in an OpScratch instead, one per stream.
 struct DevOp {
   bool present = false;
   int elem = 8;  // bytes
   DevPass p1, p2;
-  void* xt = nullptr;
   int64_t xt_size = 0;
+  int64_t xt_t = 0;     // first t entry in xt
+  int64_t t_total = 0;  // t entries
   int64_t n_in = 0, n_out = 0;
   const int64_t* perm_in = nullptr;   // device, n_in / dof
   const int64_t* perm_out = nullptr;  // device, n_out / dof
@@ -858,10 +861,72 @@ struct DevOp {
   int64_t part_elems[2] = {0, 0};
   int unroll[2] = {16, 16};   // loads in flight per lane in k_slab (4, 8 or 16), per pass
   int mfma_schedule = 0;      // k_slab_mfma / k_slab_mfma64 variant (0-7)
-  // many right-hand sides (FP32 MFMA path), allocated on first use
-  void* xb = nullptr;        // xt_size x kRhs
+};
+
+// What one matvec of an operator writes: xt (x gathered in the factor
+// precision, then t = B x), the partial sums of split work items of both
+// passes, and the same for the batched products (xt_size x kRhs input and
+// rank rows, partial sums x kRhs).  xt and xb are zero where no kernel writes
+// (the gap after x and the tail after t, which padded column groups read).
+struct OpScratch {
+  void* xt = nullptr;
+  void* part[2] = {nullptr, nullptr};
+  void* xb = nullptr;
   void* partb[2] = {nullptr, nullptr};
 };
+
+// The scratch of one stream (or of one captured graph): per operator, plus
+// the staging vectors of host-pointer calls and the full permuted output of a
+// selection.  Calls on different streams use different StreamCtx, so they
+// may run concurrently on one handle.
+struct StreamCtx {
+  hipStream_t stream = nullptr;
+  OpScratch op[2][2];  // [adjoint][fp32]
+  double* x_stage = nullptr;
+  double* y_stage = nullptr;
+  double* y_full = nullptr;
+  std::vector<void*> allocs;
+};
+
+// Allocation-failure hook for tests of the cleanup paths: with the
+// environment variable BWHIP_FAIL_ALLOC_AT=k (k >= 1), the k-th device or
+// pinned-host allocation made by one C-ABI call fails as if the memory were
+// exhausted.  The count restarts at every call that allocates.
+thread_local int64_t g_alloc_seq = 0;
+thread_local int64_t g_alloc_fail_at = 0;
+
+void arm_alloc_hook() {
+  g_alloc_seq = 0;
+  const char* e = std::getenv("BWHIP_FAIL_ALLOC_AT");
+  g_alloc_fail_at = e ? std::atoll(e) : 0;
+}
+
+bool alloc_hook_fires() { return g_alloc_fail_at > 0 && ++g_alloc_seq == g_alloc_fail_at; }
+
+// hipMalloc of at least 16 bytes, recorded in `allocs`; BW_ENOMEM on failure.
+int dev_alloc(void** p, size_t bytes, std::vector<void*>* allocs, const char* what) {
+  *p = nullptr;
+  if (alloc_hook_fires())
+    return fail(BW_ENOMEM, std::string(what) + ": hipMalloc: out of memory (BWHIP_FAIL_ALLOC_AT)");
+  const hipError_t e = hipMalloc(p, std::max<size_t>(bytes, 16));
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(BW_ENOMEM, std::string(what) + ": hipMalloc: " + hipGetErrorString(e));
+  }
+  allocs->push_back(*p);
+  return BW_OK;
+}
+
+int host_pinned_alloc(void** p, size_t bytes, const char* what) {
+  *p = nullptr;
+  if (alloc_hook_fires())
+    return fail(BW_ENOMEM, std::string(what) + ": hipHostMalloc: out of memory (BWHIP_FAIL_ALLOC_AT)");
+  if (hipHostMalloc(p, bytes) != hipSuccess) {
+    *p = nullptr;
+    return fail(BW_ENOMEM, std::string(what) + ": hipHostMalloc failed");
+  }
+  return BW_OK;
+}
 
 // Work-item bytes for the next create; 0 = sized from the matrix (auto_wi_bytes).
 std::atomic<int64_t> g_wi_bytes64{0};
@@ -883,13 +948,7 @@ int upload_pass(const bwl::Layout& L, const bwl::Program& P, DevPass* dp, hipStr
   for (int x = 0; x <= bwl::kXcds; ++x) dp->xs.s[x] = P.xcd[x];
   dp->blocks = P.xcd_blocks;
   dp->nred = static_cast<int>(P.reds.size());
-  auto dalloc = [&](void** p, size_t bytes) -> int {
-    if (bytes == 0) bytes = 16;
-    hipError_t e = hipMalloc(p, bytes);
-    if (e != hipSuccess) return fail(BW_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
-    allocs->push_back(*p);
-    return BW_OK;
-  };
+  auto dalloc = [&](void** p, size_t bytes) -> int { return dev_alloc(p, bytes, allocs, "create"); };
   int rc;
   if ((rc = dalloc(reinterpret_cast<void**>(&dp->wis), P.wis.size() * sizeof(bwl::WorkItem))))
     return rc;
@@ -902,7 +961,6 @@ int upload_pass(const bwl::Layout& L, const bwl::Program& P, DevPass* dp, hipStr
   }
   if ((rc = dalloc(reinterpret_cast<void**>(&dp->reds), P.reds.size() * sizeof(bwl::Reduce))))
     return rc;
-  if ((rc = dalloc(&dp->part, static_cast<size_t>(P.part_elems) * sizeof(T)))) return rc;
   dp->data_bytes = P.data_elems * static_cast<int64_t>(sizeof(T));
   if ((rc = dalloc(&dp->data, static_cast<size_t>(dp->data_bytes)))) return rc;
   if (!P.wis.empty())
@@ -918,11 +976,13 @@ int upload_pass(const bwl::Layout& L, const bwl::Program& P, DevPass* dp, hipStr
   for (const auto& s : P.slabs) max_slab = std::max<int64_t>(max_slab, s.ngroups * s.rows * L.opt.vec_elems);
   const int64_t chunk_elems = std::max<int64_t>(max_slab, (int64_t(256) << 20) / static_cast<int64_t>(sizeof(T)));
   T* stage[2] = {nullptr, nullptr};
-  HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&stage[0]), chunk_elems * sizeof(T)));
-  hipError_t e2 = hipHostMalloc(reinterpret_cast<void**>(&stage[1]), chunk_elems * sizeof(T));
-  if (e2 != hipSuccess) {
+  if ((rc = host_pinned_alloc(reinterpret_cast<void**>(&stage[0]), chunk_elems * sizeof(T),
+                              "create: upload staging")))
+    return rc;
+  if ((rc = host_pinned_alloc(reinterpret_cast<void**>(&stage[1]), chunk_elems * sizeof(T),
+                              "create: upload staging"))) {
     (void)hipHostFree(stage[0]);
-    return fail(BW_ENOMEM, "hipHostMalloc staging");
+    return rc;
   }
   hipEvent_t done[2] = {nullptr, nullptr};
   if (hipEventCreate(&done[0]) != hipSuccess || hipEventCreate(&done[1]) != hipSuccess) {
@@ -1014,7 +1074,6 @@ struct SelInfo {
   bool on = false;
   int ordering = BW_ORDER_ORIGINAL;
   SelDir dir[2];
-  double* y_full = nullptr;  // full permuted output of the selected blocks
 };
 
 struct bw_hip_handle {
@@ -1025,12 +1084,16 @@ struct bw_hip_handle {
   DevOp op[2][2];  // [adjoint][fp32]
   int64_t* perm0 = nullptr;
   int64_t* perm1 = nullptr;
-  double* x_stage = nullptr;
-  double* y_stage = nullptr;
   std::vector<double> diag_perm, diag_orig;
   std::vector<void*> allocs;
   int64_t device_bytes = 0;
   SelInfo sel;
+  // Scratch per stream, allocated on a stream's first call (the null stream's
+  // at creation).  stage_elems: doubles of each host-pointer staging vector;
+  // yfull_elems: doubles of a selection's full permuted output (0: none).
+  std::vector<std::unique_ptr<StreamCtx>> ctxs;
+  int64_t stage_elems = 0;
+  int64_t yfull_elems = 0;
   // Kernel timing: two alternating sets of events between the kernels of a
   // matvec; a set is read (and accumulated) only when reused two calls later,
   // so timing never stalls the launch pipeline.
@@ -1080,9 +1143,8 @@ int build_op(bw_hip_handle* hd, const bw_flat_hmat* h, bool adjoint, hipStream_t
   // 150k-DoF 3DR0 matrix (FP32: two pairs, ring of 3; FP64: ring of 4)
   op.mfma_schedule = sizeof(T) == 8 ? 5 : 4;
   op.xt_size = L.xt_size;
-  HIPCHK(hipMalloc(&op.xt, L.xt_size * sizeof(T)));
-  hd->allocs.push_back(op.xt);
-  HIPCHK(hipMemset(op.xt, 0, L.xt_size * sizeof(T)));
+  op.xt_t = L.xt_t;
+  op.t_total = L.view.t_total;
   op.n_in = static_cast<int64_t>(h->dof) * view.n_col_pts;
   op.n_out = static_cast<int64_t>(h->dof) * view.n_row_pts;
   op.perm_in = adjoint ? hd->perm0 : hd->perm1;
@@ -1096,6 +1158,94 @@ int build_op(bw_hip_handle* hd, const bw_flat_hmat* h, bool adjoint, hipStream_t
                       (L.pass1.gidx.size() + L.pass2.gidx.size()) * sizeof(int32_t);
   op.present = true;
   return BW_OK;
+}
+
+// Frees the allocations a failed ensure_* call made after `mark`.
+int undo_allocs(StreamCtx* ctx, size_t mark, int rc) {
+  for (size_t i = mark; i < ctx->allocs.size(); ++i) (void)hipFree(ctx->allocs[i]);
+  ctx->allocs.resize(mark);
+  return rc;
+}
+
+// Allocates what `ctx` lacks for operator [adj][fp32]: the single-vector
+// scratch, and with `batched` the kRhs-wide one.  xt / xb are zeroed here,
+// once (synchronously, before any kernel of the context can use them): the
+// kernels only ever write x and t entries, so the gap and tail stay zero.
+// On failure nothing of this call stays allocated.
+int ensure_op_scratch(bw_hip_handle* hd, StreamCtx* ctx, int adj, int fp32, bool batched) {
+  const DevOp& op = hd->op[adj][fp32];
+  OpScratch& s = ctx->op[adj][fp32];
+  const size_t es = op.elem;
+  const size_t mark = ctx->allocs.size();
+  int rc;
+  if (!s.xt) {
+    void* xt = nullptr;
+    void* part[2] = {nullptr, nullptr};
+    if ((rc = dev_alloc(&xt, op.xt_size * es, &ctx->allocs, "scratch"))) return undo_allocs(ctx, mark, rc);
+    for (int p = 0; p < 2; ++p)
+      if ((rc = dev_alloc(&part[p], op.part_elems[p] * es, &ctx->allocs, "scratch")))
+        return undo_allocs(ctx, mark, rc);
+    const hipError_t e = hipMemset(xt, 0, op.xt_size * es);
+    if (e != hipSuccess)
+      return undo_allocs(ctx, mark, fail(BW_EHIP, std::string("scratch: hipMemset: ") + hipGetErrorString(e)));
+    s.xt = xt;
+    s.part[0] = part[0];
+    s.part[1] = part[1];
+  }
+  if (batched && !s.xb) {
+    const size_t mark_b = ctx->allocs.size();
+    void* xb = nullptr;
+    void* partb[2] = {nullptr, nullptr};
+    if ((rc = dev_alloc(&xb, static_cast<size_t>(op.xt_size) * kRhs * es, &ctx->allocs, "scratch")))
+      return undo_allocs(ctx, mark_b, rc);
+    for (int p = 0; p < 2; ++p)
+      if ((rc = dev_alloc(&partb[p], static_cast<size_t>(op.part_elems[p]) * kRhs * es, &ctx->allocs,
+                          "scratch")))
+        return undo_allocs(ctx, mark_b, rc);
+    const hipError_t e = hipMemset(xb, 0, static_cast<size_t>(op.xt_size) * kRhs * es);
+    if (e != hipSuccess)
+      return undo_allocs(ctx, mark_b, fail(BW_EHIP, std::string("scratch: hipMemset: ") + hipGetErrorString(e)));
+    s.xb = xb;
+    s.partb[0] = partb[0];
+    s.partb[1] = partb[1];
+  }
+  return BW_OK;
+}
+
+// Staging vectors of host-pointer calls and a selection's full output.
+int ensure_ctx_vectors(bw_hip_handle* hd, StreamCtx* ctx, bool staging) {
+  const size_t mark = ctx->allocs.size();
+  int rc;
+  if (staging && !ctx->x_stage) {
+    void *xs = nullptr, *ys = nullptr;
+    if ((rc = dev_alloc(&xs, hd->stage_elems * sizeof(double), &ctx->allocs, "scratch")) ||
+        (rc = dev_alloc(&ys, hd->stage_elems * sizeof(double), &ctx->allocs, "scratch")))
+      return undo_allocs(ctx, mark, rc);
+    ctx->x_stage = static_cast<double*>(xs);
+    ctx->y_stage = static_cast<double*>(ys);
+  }
+  if (hd->yfull_elems > 0 && !ctx->y_full) {
+    const size_t mark_y = ctx->allocs.size();
+    void* yf = nullptr;
+    if ((rc = dev_alloc(&yf, hd->yfull_elems * sizeof(double), &ctx->allocs, "scratch")))
+      return undo_allocs(ctx, mark_y, rc);
+    ctx->y_full = static_cast<double*>(yf);
+  }
+  return BW_OK;
+}
+
+void free_ctx(StreamCtx* ctx) {
+  for (void* p : ctx->allocs) (void)hipFree(p);
+  ctx->allocs.clear();
+}
+
+// The context of `stream` (created empty on its first use); caller holds hd->mu.
+StreamCtx* stream_ctx(bw_hip_handle* hd, hipStream_t stream) {
+  for (auto& c : hd->ctxs)
+    if (c->stream == stream) return c.get();
+  hd->ctxs.emplace_back(new StreamCtx());
+  hd->ctxs.back()->stream = stream;
+  return hd->ctxs.back().get();
 }
 
 void compute_diagonal(bw_hip_handle* hd, const bw_flat_hmat* h) {
@@ -1138,14 +1288,32 @@ void launch_slab(int unroll, int blocks, hipStream_t st, const bwl::WorkItem* wi
                        st, wis, xs, gidx, data, xin, out, part, perm, dof, rowmap);
 }
 
+// Zero fill of xt's gap after x and its tail after t, for scratch that other
+// operators may have used before (the no-allocation entry point).
 template <typename T>
-int launch_matvec(bw_hip_handle* hd, DevOp& op, const double* xd, double* yd, int ordering,
-                  hipStream_t st, const SelDir* sd = nullptr) {
+__global__ __launch_bounds__(256) void k_zero_ranges(T* __restrict__ xt, int64_t a0, int64_t a1,
+                                                     int64_t b0, int64_t b1) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t na = a1 - a0;
+  if (i < na)
+    xt[a0 + i] = T(0);
+  else if (i - na < b1 - b0)
+    xt[b0 + (i - na)] = T(0);
+}
+
+// The five (selection: six) kernels of one matvec on stream st, writing only
+// the scratch `s` (and y_full for a selection).
+template <typename T>
+int launch_matvec(bw_hip_handle* hd, DevOp& op, const OpScratch& s, double* y_full,
+                  const double* xd, double* yd, int ordering, hipStream_t st,
+                  const SelDir* sd = nullptr) {
   const bool orig = !sd && ordering == BW_ORDER_ORIGINAL;
   const int dof = hd->dof;
-  T* xt = static_cast<T*>(op.xt);
+  T* xt = static_cast<T*>(s.xt);
+  T* part1 = static_cast<T*>(s.part[0]);
+  T* part2 = static_cast<T*>(s.part[1]);
   double* const y_user = yd;
-  if (sd) yd = hd->sel.y_full;  // the passes write the full permuted y
+  if (sd) yd = y_full;  // the passes write the full permuted y
   if (hd->timing) (void)hipEventRecord(hd->ev[0], st);
   if (sd) {
     if (sd->n_pairs > 0) {
@@ -1162,26 +1330,23 @@ int launch_matvec(bw_hip_handle* hd, DevOp& op, const double* xd, double* yd, in
   if (hd->timing) (void)hipEventRecord(hd->ev[1], st);
   if (op.p1.nwi > 0) {
     launch_slab<T, T, 1>(op.unroll[0], op.p1.blocks, st, op.p1.wis, op.p1.xs, op.p1.gidx,
-                         static_cast<const T*>(op.p1.data), xt, xt, static_cast<T*>(op.p1.part),
-                         nullptr, dof, op.p1.rowmap);
+                         static_cast<const T*>(op.p1.data), xt, xt, part1, nullptr, dof,
+                         op.p1.rowmap);
   }
   if (hd->timing) (void)hipEventRecord(hd->ev[2], st);
   if (op.p1.nred > 0)
     hipLaunchKernelGGL((k_reduce<T, T>), dim3(op.p1.nred), dim3(64), 0, st, op.p1.reds,
-                       op.p1.nred, static_cast<const T*>(op.p1.part), xt, nullptr, dof,
-                       op.p1.rowmap);
+                       op.p1.nred, part1, xt, nullptr, dof, op.p1.rowmap);
   if (hd->timing) (void)hipEventRecord(hd->ev[3], st);
   if (op.p2.nwi > 0) {
     launch_slab<T, double, 2>(op.unroll[1], op.p2.blocks, st, op.p2.wis, op.p2.xs, op.p2.gidx,
-                              static_cast<const T*>(op.p2.data), xt, yd,
-                              static_cast<T*>(op.p2.part), orig ? op.perm_out : nullptr, dof,
-                              nullptr);
+                              static_cast<const T*>(op.p2.data), xt, yd, part2,
+                              orig ? op.perm_out : nullptr, dof, nullptr);
   }
   if (hd->timing) (void)hipEventRecord(hd->ev[4], st);
   if (op.p2.nred > 0)
     hipLaunchKernelGGL((k_reduce<T, double>), dim3(op.p2.nred), dim3(64), 0, st, op.p2.reds,
-                       op.p2.nred, static_cast<const T*>(op.p2.part), yd,
-                       orig ? op.perm_out : nullptr, dof, nullptr);
+                       op.p2.nred, part2, yd, orig ? op.perm_out : nullptr, dof, nullptr);
   if (sd && sd->n_out > 0) {
     const int blocks = static_cast<int>((sd->n_out * dof + 255) / 256);
     hipLaunchKernelGGL(k_gather_sel, dim3(blocks), dim3(256), 0, st, yd, y_user, sd->out_dst,
@@ -1214,7 +1379,8 @@ int do_matvec(bw_hip_handle* hd, bool adjoint, const double* x, double* y, int o
     return fail(BW_EINVAL, "matvec: ordering must be 0 (permuted) or 1 (original)");
   if (precision != BW_PREC_FP64 && precision != BW_PREC_FP32)
     return fail(BW_EINVAL, "matvec: precision must be BW_PREC_FP64 or BW_PREC_FP32");
-  DevOp& op = hd->op[adjoint ? 1 : 0][precision == BW_PREC_FP32 ? 1 : 0];
+  const int adj = adjoint ? 1 : 0, f32 = precision == BW_PREC_FP32 ? 1 : 0;
+  DevOp& op = hd->op[adj][f32];
   if (!op.present)
     return fail(BW_ENOTSUP, adjoint ? "rmatvec: adjoint / precision not built for this handle"
                                     : "matvec: precision not built for this handle");
@@ -1222,19 +1388,25 @@ int do_matvec(bw_hip_handle* hd, bool adjoint, const double* x, double* y, int o
   if (hd->sel.on) {
     if (ordering != hd->sel.ordering)
       return fail(BW_ENOTSUP, "matvec: this selection was created for the other ordering");
-    sd = &hd->sel.dir[adjoint ? 1 : 0];
+    sd = &hd->sel.dir[adj];
   }
   const int64_t n_in = sd ? hd->dof * sd->n_in : op.n_in;
   const int64_t n_out = sd ? hd->dof * sd->n_out : op.n_out;
   std::lock_guard<std::mutex> lock(hd->mu);
+  arm_alloc_hook();
   hipStream_t st = static_cast<hipStream_t>(stream);
   HIPCHK(hipSetDevice(hd->device));
+  StreamCtx* ctx = stream_ctx(hd, st);
+  int rc;
+  if ((rc = ensure_op_scratch(hd, ctx, adj, f32, false)) ||
+      (rc = ensure_ctx_vectors(hd, ctx, !on_device)))
+    return rc;
   const double* xd = x;
   double* yd = y;
   if (!on_device) {
-    HIPCHK(hipMemcpyAsync(hd->x_stage, x, n_in * sizeof(double), hipMemcpyHostToDevice, st));
-    xd = hd->x_stage;
-    yd = hd->y_stage;
+    HIPCHK(hipMemcpyAsync(ctx->x_stage, x, n_in * sizeof(double), hipMemcpyHostToDevice, st));
+    xd = ctx->x_stage;
+    yd = ctx->y_stage;
   }
   if (hd->timing) {
     const int set = hd->cur;
@@ -1244,15 +1416,16 @@ int do_matvec(bw_hip_handle* hd, bool adjoint, const double* x, double* y, int o
     }
     hd->ev = hd->evs[set];
   }
-  int rc = precision == BW_PREC_FP64 ? launch_matvec<double>(hd, op, xd, yd, ordering, st, sd)
-                                     : launch_matvec<float>(hd, op, xd, yd, ordering, st, sd);
+  const OpScratch& s = ctx->op[adj][f32];
+  rc = f32 ? launch_matvec<float>(hd, op, s, ctx->y_full, xd, yd, ordering, st, sd)
+           : launch_matvec<double>(hd, op, s, ctx->y_full, xd, yd, ordering, st, sd);
   if (rc) return rc;
   if (hd->timing) {
     hd->pending[hd->cur] = true;
     hd->cur ^= 1;
   }
   if (!on_device) {
-    HIPCHK(hipMemcpyAsync(y, hd->y_stage, n_out * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(y, ctx->y_stage, n_out * sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
   }
   return BW_OK;
@@ -1281,6 +1454,7 @@ void bw_hip_destroy(bw_hip_handle* hd) {
   if (!hd) return;
   (void)hipSetDevice(hd->device);
   (void)hipDeviceSynchronize();
+  for (auto& c : hd->ctxs) free_ctx(c.get());
   for (void* p : hd->allocs) (void)hipFree(p);
   for (auto& set : hd->evs)
     for (auto& e : set)
@@ -1296,8 +1470,19 @@ int compose_flat(const bw_flat_hmat* const* hs, int n, bw_flat_composite* c, std
 
 namespace {
 
+// The null stream's single-vector scratch and staging, allocated up front
+// (most callers use that stream; other streams get theirs on first use).
+int prepare_null_stream(bw_hip_handle* hd) {
+  StreamCtx* ctx0 = stream_ctx(hd, nullptr);
+  int rc;
+  for (int adj = 0; adj < 2; ++adj)
+    for (int f32 = 0; f32 < 2; ++f32)
+      if (hd->op[adj][f32].present && (rc = ensure_op_scratch(hd, ctx0, adj, f32, false))) return rc;
+  return ensure_ctx_vectors(hd, ctx0, true);
+}
+
 int create_impl(const bw_flat_hmat* h, long double expected_area, int precision_mask, int flags,
-                int device, bw_hip_handle** out) {
+                int device, bw_hip_handle** out, bool prepare_ctx0 = true) {
   if (!out) return fail(BW_EINVAL, "create: null output pointer");
   *out = nullptr;
   std::string msg;
@@ -1325,17 +1510,10 @@ int create_impl(const bw_flat_hmat* h, long double expected_area, int precision_
   hd->n_row_pts = h->n_row_pts;
   hd->n_col_pts = h->n_col_pts;
   const int64_t d = h->dof;
-  auto dmal = [&](void** p, size_t bytes) -> int {
-    hipError_t e = hipMalloc(p, std::max<size_t>(bytes, 16));
-    if (e != hipSuccess) return fail(BW_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
-    hd->allocs.push_back(*p);
-    return BW_OK;
-  };
+  auto dmal = [&](void** p, size_t bytes) -> int { return dev_alloc(p, bytes, &hd->allocs, "create"); };
   if ((rc = dmal(reinterpret_cast<void**>(&hd->perm0), h->n_row_pts * sizeof(int64_t)))) return cleanup(rc);
   if ((rc = dmal(reinterpret_cast<void**>(&hd->perm1), h->n_col_pts * sizeof(int64_t)))) return cleanup(rc);
-  const int64_t nmax = d * std::max(h->n_row_pts, h->n_col_pts);
-  if ((rc = dmal(reinterpret_cast<void**>(&hd->x_stage), nmax * sizeof(double)))) return cleanup(rc);
-  if ((rc = dmal(reinterpret_cast<void**>(&hd->y_stage), nmax * sizeof(double)))) return cleanup(rc);
+  hd->stage_elems = d * std::max(h->n_row_pts, h->n_col_pts);
   if (hipMemcpy(hd->perm0, h->perm0, h->n_row_pts * sizeof(int64_t), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(hd->perm1, h->perm1, h->n_col_pts * sizeof(int64_t), hipMemcpyHostToDevice) != hipSuccess)
     return cleanup(fail(BW_EHIP, "create: permutation upload"));
@@ -1374,22 +1552,27 @@ int create_impl(const bw_flat_hmat* h, long double expected_area, int precision_
   for (auto& set : hd->evs)
     for (auto& e : set)
       if (hipEventCreate(&e) != hipSuccess) return cleanup(fail(BW_EHIP, "create: hipEventCreate"));
+  if (prepare_ctx0 && (rc = prepare_null_stream(hd))) return cleanup(rc);
   *out = hd;
   return BW_OK;
 }
 
 }  // namespace
 
+// A captured matvec.  It owns its scratch, so replays never race with
+// eager calls on the handle (two replays of one graph must be ordered).
 struct bw_hip_graph {
   int device = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  StreamCtx ctx;
 };
 
 extern "C" {
 
 int bw_hip_create(const bw_flat_hmat* h, int precision_mask, int flags, int device,
                   bw_hip_handle** out) {
+  arm_alloc_hook();
   return create_impl(h, -1.0L, precision_mask, flags, device, out);
 }
 
@@ -1397,6 +1580,7 @@ int bw_hip_create_multi(const bw_flat_hmat* const* hs, int n_mats, int precision
                         int device, bw_hip_handle** out, int64_t* row_off, int64_t* col_off) {
   if (!out) return fail(BW_EINVAL, "create_multi: null output pointer");
   *out = nullptr;
+  arm_alloc_hook();
   bw_flat_composite* c = nullptr;
   int rc = bw_flat_compose(hs, n_mats, &c);
   if (rc) return fail(rc, std::string("create_multi: ") + bw_flat_compose_last_error());
@@ -1415,6 +1599,7 @@ int bw_hip_create_selection(const bw_flat_hmat* h, const int64_t* rows, int64_t 
                             int precision_mask, int flags, int device, bw_hip_handle** out) {
   if (!out) return fail(BW_EINVAL, "create_selection: null output pointer");
   *out = nullptr;
+  arm_alloc_hook();
   std::string msg;
   int rc = bwl::validate_flat(h, &msg);
   if (rc) return fail(rc, "create_selection: " + msg);
@@ -1477,7 +1662,7 @@ int bw_hip_create_selection(const bw_flat_hmat* h, const int64_t* rows, int64_t 
   sub.lr_A = sub.n_lr ? lr_a.data() : nullptr;
   sub.lr_B = sub.n_lr ? lr_b.data() : nullptr;
   bw_hip_handle* hd = nullptr;
-  rc = create_impl(&sub, area, precision_mask, flags, device, &hd);
+  rc = create_impl(&sub, area, precision_mask, flags, device, &hd, false);
   if (rc) return rc;
   // scatter pairs with repeated targets reduced to the last occurrence, as
   // HmatSelection::matvec overwrites x_full (hmat_selection.cc:197-205)
@@ -1493,10 +1678,9 @@ int bw_hip_create_selection(const bw_flat_hmat* h, const int64_t* rows, int64_t 
       }
   };
   auto up = [&](const std::vector<int64_t>& v, int64_t** dp) -> int {
-    if (hipMalloc(reinterpret_cast<void**>(dp), std::max<size_t>(v.size(), 1) * sizeof(int64_t)) !=
-        hipSuccess)
-      return fail(BW_ENOMEM, "create_selection: hipMalloc");
-    hd->allocs.push_back(*dp);
+    int rc1 = dev_alloc(reinterpret_cast<void**>(dp), v.size() * sizeof(int64_t), &hd->allocs,
+                        "create_selection");
+    if (rc1) return rc1;
     if (!v.empty() &&
         hipMemcpy(*dp, v.data(), v.size() * sizeof(int64_t), hipMemcpyHostToDevice) != hipSuccess)
       return fail(BW_EHIP, "create_selection: upload");
@@ -1517,23 +1701,15 @@ int bw_hip_create_selection(const bw_flat_hmat* h, const int64_t* rows, int64_t 
     sd.n_in = static_cast<int64_t>(in_of[dir]->size());
     sd.n_out = static_cast<int64_t>(out_of[dir]->size());
   }
+  // every stream's scratch holds the full permuted output of the selected
+  // blocks; host-pointer staging must hold the selection vectors, which
+  // repeated indices can make longer than the full ones
   const int64_t nfull = h->dof * std::max(h->n_row_pts, h->n_col_pts);
-  if (hipMalloc(reinterpret_cast<void**>(&hd->sel.y_full), nfull * sizeof(double)) != hipSuccess) {
+  hd->yfull_elems = nfull;
+  hd->stage_elems = std::max(nfull, h->dof * std::max(n_rows, n_cols));
+  if ((rc = prepare_null_stream(hd))) {
     bw_hip_destroy(hd);
-    return fail(BW_ENOMEM, "create_selection: hipMalloc");
-  }
-  hd->allocs.push_back(hd->sel.y_full);
-  // host-pointer staging must hold the selection vectors, which repeated
-  // indices can make longer than the full ones
-  const int64_t nsel = h->dof * std::max(n_rows, n_cols);
-  if (nsel > nfull) {
-    for (double** stage : {&hd->x_stage, &hd->y_stage}) {
-      if (hipMalloc(reinterpret_cast<void**>(stage), nsel * sizeof(double)) != hipSuccess) {
-        bw_hip_destroy(hd);
-        return fail(BW_ENOMEM, "create_selection: hipMalloc");
-      }
-      hd->allocs.push_back(*stage);
-    }
+    return rc;
   }
   hd->sel.on = true;
   hd->sel.ordering = ordering;
@@ -1541,73 +1717,6 @@ int bw_hip_create_selection(const bw_flat_hmat* h, const int64_t* rows, int64_t 
   hd->n_col_pts = n_cols;
   *out = hd;
   return BW_OK;
-}
-
-int bw_hip_graph_create(bw_hip_handle* hd, const double* x, double* y, int ordering,
-                        int precision, int adjoint, bw_hip_graph** out) {
-  if (!hd || !x || !y || !out) return fail(BW_EINVAL, "graph_create: null argument");
-  *out = nullptr;
-  if (ordering != BW_ORDER_PERMUTED && ordering != BW_ORDER_ORIGINAL)
-    return fail(BW_EINVAL, "graph_create: ordering must be 0 (permuted) or 1 (original)");
-  if (precision != BW_PREC_FP64 && precision != BW_PREC_FP32)
-    return fail(BW_EINVAL, "graph_create: precision must be BW_PREC_FP64 or BW_PREC_FP32");
-  DevOp& op = hd->op[adjoint ? 1 : 0][precision == BW_PREC_FP32 ? 1 : 0];
-  if (!op.present) return fail(BW_ENOTSUP, "graph_create: adjoint / precision not built");
-  std::lock_guard<std::mutex> lock(hd->mu);
-  HIPCHK(hipSetDevice(hd->device));
-  hipStream_t cs;
-  HIPCHK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
-  const bool timing = hd->timing;
-  hd->timing = false;  // no event records inside the graph
-  hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
-  int rc = BW_OK;
-  hipGraph_t graph = nullptr;
-  if (e == hipSuccess) {
-    const SelDir* sd = hd->sel.on ? &hd->sel.dir[adjoint ? 1 : 0] : nullptr;
-    if (sd && ordering != hd->sel.ordering) {
-      rc = fail(BW_ENOTSUP, "graph_create: this selection was created for the other ordering");
-    } else {
-      rc = precision == BW_PREC_FP64 ? launch_matvec<double>(hd, op, x, y, ordering, cs, sd)
-                                     : launch_matvec<float>(hd, op, x, y, ordering, cs, sd);
-    }
-    hipError_t e2 = hipStreamEndCapture(cs, &graph);
-    if (!rc && e2 != hipSuccess) rc = fail(BW_EHIP, std::string("graph_create: ") + hipGetErrorString(e2));
-  } else {
-    rc = fail(BW_EHIP, std::string("graph_create: ") + hipGetErrorString(e));
-  }
-  hd->timing = timing;
-  (void)hipStreamDestroy(cs);
-  if (rc) {
-    if (graph) (void)hipGraphDestroy(graph);
-    return rc;
-  }
-  auto* g = new bw_hip_graph();
-  g->device = hd->device;
-  g->graph = graph;
-  e = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
-  if (e != hipSuccess) {
-    (void)hipGraphDestroy(graph);
-    delete g;
-    return fail(BW_EHIP, std::string("graph_create: instantiate: ") + hipGetErrorString(e));
-  }
-  *out = g;
-  return BW_OK;
-}
-
-int bw_hip_graph_launch(bw_hip_graph* g, void* stream) {
-  if (!g) return fail(BW_EINVAL, "graph_launch: null graph");
-  HIPCHK(hipSetDevice(g->device));
-  HIPCHK(hipGraphLaunch(g->exec, static_cast<hipStream_t>(stream)));
-  return BW_OK;
-}
-
-void bw_hip_graph_destroy(bw_hip_graph* g) {
-  if (!g) return;
-  (void)hipSetDevice(g->device);
-  (void)hipDeviceSynchronize();
-  if (g->exec) (void)hipGraphExecDestroy(g->exec);
-  if (g->graph) (void)hipGraphDestroy(g->graph);
-  delete g;
 }
 
 int bw_hip_matvec(bw_hip_handle* hd, const double* x, double* y, int ordering, int precision,
@@ -1624,11 +1733,12 @@ int bw_hip_rmatvec(bw_hip_handle* hd, const double* x, double* y, int ordering, 
 
 namespace {
 
+// Checks of the batched entry points; on success *op is the operator used.
 // X(i, j) = X[i * xr + j * xc] (input index i, right-hand side j); Y likewise
 // with (yr, yc).  Column-major is (1, ld), row-major (nrhs, 1).
-int batched_impl(bw_hip_handle* hd, bool adjoint, int nrhs, const double* X, int64_t xr,
-                 int64_t xc, double* Y, int64_t yr, int64_t yc, int ordering, int precision,
-                 void* stream, bool leading_dims = false) {
+int check_batched(bw_hip_handle* hd, bool adjoint, int nrhs, const double* X, int64_t xr,
+                  int64_t xc, const double* Y, int64_t yr, int64_t yc, int ordering, int precision,
+                  bool leading_dims, DevOp** op_out) {
   if (!hd || !X || !Y || nrhs <= 0) return fail(BW_EINVAL, "matvec_batched: bad argument");
   if (xr < 1 || xc < 1 || yr < 1 || yc < 1)
     return fail(BW_ESHAPE, "matvec_batched: strides must be positive");
@@ -1647,20 +1757,15 @@ int batched_impl(bw_hip_handle* hd, bool adjoint, int nrhs, const double* X, int
   // column's extent, or rows apart by at least a row's
   if (!(yc >= op.n_out * yr || yr >= static_cast<int64_t>(nrhs) * yc))
     return fail(BW_ESHAPE, "matvec_batched: output strides overlap");
-  std::lock_guard<std::mutex> lock(hd->mu);
-  HIPCHK(hipSetDevice(hd->device));
-  hipStream_t st = static_cast<hipStream_t>(stream);
-  const bool fp64 = precision == BW_PREC_FP64;
-  const size_t es = fp64 ? sizeof(double) : sizeof(float);
-  if (!op.xb) {
-    HIPCHK(hipMalloc(&op.xb, static_cast<size_t>(op.xt_size) * kRhs * es));
-    hd->allocs.push_back(op.xb);
-    HIPCHK(hipMemset(op.xb, 0, static_cast<size_t>(op.xt_size) * kRhs * es));
-    for (int p = 0; p < 2; ++p) {
-      HIPCHK(hipMalloc(&op.partb[p], std::max<int64_t>(op.part_elems[p], 1) * kRhs * es));
-      hd->allocs.push_back(op.partb[p]);
-    }
-  }
+  *op_out = &op;
+  return BW_OK;
+}
+
+// The kernels of Y = H X (kRhs right-hand sides per round) on stream st,
+// writing only the batched scratch `s`.
+int launch_batched(bw_hip_handle* hd, DevOp& op, const OpScratch& s, int nrhs, const double* X,
+                   int64_t xr, int64_t xc, double* Y, int64_t yr, int64_t yc, int ordering,
+                   bool fp64, hipStream_t st) {
   const bool orig = ordering == BW_ORDER_ORIGINAL;
   const int dof = hd->dof;
   for (int k0 = 0; k0 < nrhs; k0 += kRhs) {
@@ -1672,9 +1777,9 @@ int batched_impl(bw_hip_handle* hd, bool adjoint, int nrhs, const double* X, int
     const int64_t* pin = orig ? op.perm_in : nullptr;
     const int64_t* pout = orig ? op.perm_out : nullptr;
     if (fp64) {
-      double* xb = static_cast<double*>(op.xb);
-      double* pb0 = static_cast<double*>(op.partb[0]);
-      double* pb1 = static_cast<double*>(op.partb[1]);
+      double* xb = static_cast<double*>(s.xb);
+      double* pb0 = static_cast<double*>(s.partb[0]);
+      double* pb1 = static_cast<double*>(s.partb[1]);
       hipLaunchKernelGGL(k_gather_b<double>, dim3(gblocks), dim3(256), 0, st, Xk, xr, xc, nk, xb,
                          op.n_in, pin, dof);
       if (op.p1.nwi > 0)
@@ -1692,9 +1797,9 @@ int batched_impl(bw_hip_handle* hd, bool adjoint, int nrhs, const double* X, int
         hipLaunchKernelGGL((k_reduce_b<2, double>), dim3(op.p2.nred), dim3(256), 0, st, op.p2.reds,
                            op.p2.nred, pb1, nullptr, Yk, yr, yc, nk, pout, dof, nullptr);
     } else {
-      float* xb = static_cast<float*>(op.xb);
-      float* pb0 = static_cast<float*>(op.partb[0]);
-      float* pb1 = static_cast<float*>(op.partb[1]);
+      float* xb = static_cast<float*>(s.xb);
+      float* pb0 = static_cast<float*>(s.partb[0]);
+      float* pb1 = static_cast<float*>(s.partb[1]);
       hipLaunchKernelGGL(k_gather_b<float>, dim3(gblocks), dim3(256), 0, st, Xk, xr, xc, nk, xb,
                          op.n_in, pin, dof);
       if (op.p1.nwi > 0)
@@ -1717,9 +1822,155 @@ int batched_impl(bw_hip_handle* hd, bool adjoint, int nrhs, const double* X, int
   return BW_OK;
 }
 
+int batched_impl(bw_hip_handle* hd, bool adjoint, int nrhs, const double* X, int64_t xr,
+                 int64_t xc, double* Y, int64_t yr, int64_t yc, int ordering, int precision,
+                 void* stream, bool leading_dims = false) {
+  DevOp* op = nullptr;
+  int rc = check_batched(hd, adjoint, nrhs, X, xr, xc, Y, yr, yc, ordering, precision,
+                         leading_dims, &op);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lock(hd->mu);
+  arm_alloc_hook();
+  HIPCHK(hipSetDevice(hd->device));
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const int adj = adjoint ? 1 : 0, f32 = precision == BW_PREC_FP32 ? 1 : 0;
+  StreamCtx* ctx = stream_ctx(hd, st);
+  if ((rc = ensure_op_scratch(hd, ctx, adj, f32, true))) return rc;
+  return launch_batched(hd, *op, ctx->op[adj][f32], nrhs, X, xr, xc, Y, yr, yc, ordering, !f32, st);
+}
+
+// Captures launch(stream, ctx) on a private stream into a new graph that owns
+// ctx (already holding its scratch).  Caller holds hd->mu.
+template <typename F>
+int capture_graph(bw_hip_handle* hd, std::unique_ptr<bw_hip_graph> g, F launch,
+                  bw_hip_graph** out) {
+  hipStream_t cs;
+  const hipError_t ec = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+  if (ec != hipSuccess) {
+    free_ctx(&g->ctx);
+    return fail(BW_EHIP, std::string("graph_create: ") + hipGetErrorString(ec));
+  }
+  const bool timing = hd->timing;
+  hd->timing = false;  // no event records inside the graph
+  hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+  int rc = BW_OK;
+  hipGraph_t graph = nullptr;
+  if (e == hipSuccess) {
+    rc = launch(cs, g->ctx);
+    hipError_t e2 = hipStreamEndCapture(cs, &graph);
+    if (!rc && e2 != hipSuccess) rc = fail(BW_EHIP, std::string("graph_create: ") + hipGetErrorString(e2));
+  } else {
+    rc = fail(BW_EHIP, std::string("graph_create: ") + hipGetErrorString(e));
+  }
+  hd->timing = timing;
+  (void)hipStreamDestroy(cs);
+  if (!rc) {
+    g->graph = graph;
+    e = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) rc = fail(BW_EHIP, std::string("graph_create: instantiate: ") + hipGetErrorString(e));
+  }
+  if (rc) {
+    if (graph) (void)hipGraphDestroy(graph);
+    free_ctx(&g->ctx);
+    return rc;
+  }
+  *out = g.release();
+  return BW_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+int bw_hip_graph_create(bw_hip_handle* hd, const double* x, double* y, int ordering,
+                        int precision, int adjoint, bw_hip_graph** out) {
+  if (!hd || !x || !y || !out) return fail(BW_EINVAL, "graph_create: null argument");
+  *out = nullptr;
+  if (ordering != BW_ORDER_PERMUTED && ordering != BW_ORDER_ORIGINAL)
+    return fail(BW_EINVAL, "graph_create: ordering must be 0 (permuted) or 1 (original)");
+  if (precision != BW_PREC_FP64 && precision != BW_PREC_FP32)
+    return fail(BW_EINVAL, "graph_create: precision must be BW_PREC_FP64 or BW_PREC_FP32");
+  const int adj = adjoint ? 1 : 0, f32 = precision == BW_PREC_FP32 ? 1 : 0;
+  DevOp& op = hd->op[adj][f32];
+  if (!op.present) return fail(BW_ENOTSUP, "graph_create: adjoint / precision not built");
+  const SelDir* sd = hd->sel.on ? &hd->sel.dir[adj] : nullptr;
+  if (sd && ordering != hd->sel.ordering)
+    return fail(BW_ENOTSUP, "graph_create: this selection was created for the other ordering");
+  std::lock_guard<std::mutex> lock(hd->mu);
+  arm_alloc_hook();
+  HIPCHK(hipSetDevice(hd->device));
+  std::unique_ptr<bw_hip_graph> g(new bw_hip_graph());
+  g->device = hd->device;
+  int rc;
+  if ((rc = ensure_op_scratch(hd, &g->ctx, adj, f32, false)) ||
+      (rc = ensure_ctx_vectors(hd, &g->ctx, false))) {
+    free_ctx(&g->ctx);
+    return rc;
+  }
+  return capture_graph(hd, std::move(g), [&](hipStream_t cs, StreamCtx& ctx) {
+    const OpScratch& s = ctx.op[adj][f32];
+    return f32 ? launch_matvec<float>(hd, op, s, ctx.y_full, x, y, ordering, cs, sd)
+               : launch_matvec<double>(hd, op, s, ctx.y_full, x, y, ordering, cs, sd);
+  }, out);
+}
+
+int bw_hip_graph_create_batched(bw_hip_handle* hd, int adjoint, int nrhs, const double* X,
+                                int64_t x_row_stride, int64_t x_col_stride, double* Y,
+                                int64_t y_row_stride, int64_t y_col_stride, int ordering,
+                                int precision, bw_hip_graph** out) {
+  if (!out) return fail(BW_EINVAL, "graph_create_batched: null argument");
+  *out = nullptr;
+  DevOp* op = nullptr;
+  int rc = check_batched(hd, adjoint != 0, nrhs, X, x_row_stride, x_col_stride, Y, y_row_stride,
+                         y_col_stride, ordering, precision, false, &op);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lock(hd->mu);
+  arm_alloc_hook();
+  HIPCHK(hipSetDevice(hd->device));
+  const int adj = adjoint ? 1 : 0, f32 = precision == BW_PREC_FP32 ? 1 : 0;
+  std::unique_ptr<bw_hip_graph> g(new bw_hip_graph());
+  g->device = hd->device;
+  if ((rc = ensure_op_scratch(hd, &g->ctx, adj, f32, true))) {
+    free_ctx(&g->ctx);
+    return rc;
+  }
+  return capture_graph(hd, std::move(g), [&](hipStream_t cs, StreamCtx& ctx) {
+    return launch_batched(hd, *op, ctx.op[adj][f32], nrhs, X, x_row_stride, x_col_stride, Y,
+                          y_row_stride, y_col_stride, ordering, !f32, cs);
+  }, out);
+}
+
+int bw_hip_graph_launch(bw_hip_graph* g, void* stream) {
+  if (!g) return fail(BW_EINVAL, "graph_launch: null graph");
+  HIPCHK(hipSetDevice(g->device));
+  HIPCHK(hipGraphLaunch(g->exec, static_cast<hipStream_t>(stream)));
+  return BW_OK;
+}
+
+void bw_hip_graph_destroy(bw_hip_graph* g) {
+  if (!g) return;
+  (void)hipSetDevice(g->device);
+  (void)hipDeviceSynchronize();
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  free_ctx(&g->ctx);
+  delete g;
+}
+
+int bw_hip_stream_release(bw_hip_handle* hd, void* stream) {
+  if (!hd) return fail(BW_EINVAL, "stream_release: null handle");
+  std::lock_guard<std::mutex> lock(hd->mu);
+  HIPCHK(hipSetDevice(hd->device));
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  for (size_t i = 0; i < hd->ctxs.size(); ++i) {
+    if (hd->ctxs[i]->stream != st) continue;
+    HIPCHK(hipStreamSynchronize(st));
+    free_ctx(hd->ctxs[i].get());
+    hd->ctxs.erase(hd->ctxs.begin() + static_cast<std::ptrdiff_t>(i));
+    return BW_OK;
+  }
+  return BW_OK;
+}
 
 int bw_hip_matvec_batched(bw_hip_handle* hd, int nrhs, const double* X, int64_t ldx, double* Y,
                           int64_t ldy, int ordering, int precision, void* stream) {

@@ -24,7 +24,8 @@ int bw_flat_full_blocks(const bw_flat_hmat* h, int ordering, const int64_t* rows
                         const int64_t* cols, int64_t n_cols, int64_t* count, double* values,
                         int64_t* row_idx, int64_t* col_idx) {
   std::string msg;
-  int rc = bwl::validate_flat(h, &msg);
+  // only the full-rank blocks are read: the low-rank factors may be gone
+  int rc = bwl::validate_flat(h, &msg, -1.0L, false);
   if (rc) {
     g_ferr = msg;
     return rc;

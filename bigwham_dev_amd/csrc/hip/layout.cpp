@@ -20,7 +20,8 @@ int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
 
 }  // namespace
 
-int validate_flat(const bw_flat_hmat* h, std::string* msg, long double expected_area) {
+int validate_flat(const bw_flat_hmat* h, std::string* msg, long double expected_area,
+                  bool need_lr_data) {
   if (!h) {
     *msg = "null bw_flat_hmat";
     return BW_EINVAL;
@@ -34,7 +35,7 @@ int validate_flat(const bw_flat_hmat* h, std::string* msg, long double expected_
     return BW_EINVAL;
   }
   if (h->n_fr < 0 || h->n_lr < 0 || (h->n_fr > 0 && (!h->fr_pattern || !h->fr_blocks)) ||
-      (h->n_lr > 0 && (!h->lr_pattern || !h->lr_A || !h->lr_B))) {
+      (h->n_lr > 0 && (!h->lr_pattern || (need_lr_data && (!h->lr_A || !h->lr_B))))) {
     *msg = "missing pattern or block pointers";
     return BW_EINVAL;
   }
@@ -71,7 +72,7 @@ int validate_flat(const bw_flat_hmat* h, std::string* msg, long double expected_
     const int64_t i1 = PAT(h->lr_pattern, 3, k), j1 = PAT(h->lr_pattern, 4, k);
     const int64_t r = PAT(h->lr_pattern, 5, k);
     if (i0 < 0 || j0 < 0 || i1 <= i0 || j1 <= j0 || i1 > h->n_row_pts || j1 > h->n_col_pts ||
-        r < 0 || (r > 0 && (!h->lr_A[k] || !h->lr_B[k]))) {
+        r < 0 || (need_lr_data && r > 0 && (!h->lr_A[k] || !h->lr_B[k]))) {
       *msg = "low-rank block " + std::to_string(k) + " out of range";
       return BW_ESHAPE;
     }

@@ -158,7 +158,10 @@ struct Layout {
 // Validates the flat H-matrix; returns BW_OK or an error code with message.
 // The blocks must cover `expected_area` point pairs: the whole matrix by
 // default, the diagonal blocks' total for a block-diagonal composite.
-int validate_flat(const bw_flat_hmat* h, std::string* msg, long double expected_area = -1.0L);
+// need_lr_data = false accepts missing low-rank factor pointers (lr_A / lr_B
+// NULL, or NULL entries) for callers that read only the full-rank blocks.
+int validate_flat(const bw_flat_hmat* h, std::string* msg, long double expected_area = -1.0L,
+                  bool need_lr_data = true);
 
 void make_view(const bw_flat_hmat* h, bool transposed, OpView* v);
 void build_layout(const OpView& v, const LayoutOptions& opt, Layout* L);

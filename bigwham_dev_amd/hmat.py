@@ -361,6 +361,20 @@ class HipHmat:
         """Capture y = H x (or H^T x) on these device tensors as one HIP graph."""
         return MatvecGraph(self, x, y, original, precision, adjoint)
 
+    def graph_batched(self, X, Y=None, original: bool = True, precision: str = "fp64",
+                      adjoint: bool = False) -> "MatvecGraph":
+        """Capture Y = H X (or H^T X) for an (n, k) float64 CUDA tensor X (any
+        positive strides, as matvec_batched) as one HIP graph
+        (bw_hip_graph_create_batched); Y defaults to a row-major (n_out, k)."""
+        return MatvecGraph(self, X, Y, original, precision, adjoint, batched=True)
+
+    def release_stream(self, stream: int | None = None) -> None:
+        """Free the device scratch kept for a stream (bw_hip_stream_release)."""
+        import torch
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        N.check_hip(N.hip().bw_hip_stream_release(self._h, C.c_void_p(stream)))
+
     def set_unroll(self, fp64: int = 0, fp32: int = 0) -> None:
         N.check_hip(N.hip().bw_hip_set_unroll(self._h, int(fp64), int(fp32)))
 
@@ -390,30 +404,48 @@ class HipHmat:
 
 
 class MatvecGraph:
-    """A matvec on fixed device vectors, replayed with one graph launch
-    (bw_hip_graph_create / bw_hip_graph_launch).  Shares the handle's
-    scratch: its launches must not overlap other calls on the same handle."""
+    """A matvec (or a batched product) on fixed device operands, replayed with
+    one graph launch (bw_hip_graph_create[_batched] / bw_hip_graph_launch).
+    The graph owns its scratch; two replays of one graph must be ordered."""
 
-    def __init__(self, dev: HipHmat, x, y, original, precision, adjoint):
+    def __init__(self, dev: HipHmat, x, y, original, precision, adjoint, batched=False):
         import torch
-        if not (isinstance(x, torch.Tensor) and x.is_cuda and x.dtype == torch.float64
-                and x.is_contiguous()):
-            raise TypeError("x must be a contiguous float64 CUDA tensor")
         n_in, n_out = (dev.n_rows, dev.n_cols) if adjoint else (dev.n_cols, dev.n_rows)
-        if x.numel() != n_in:
-            raise ValueError(f"graph: x has {x.numel()} entries, the operator takes {n_in}")
-        if y is None:
-            y = torch.empty(n_out, dtype=torch.float64, device=x.device)
-        if (not isinstance(y, torch.Tensor) or y.dtype != torch.float64 or y.device != x.device
-                or y.numel() != n_out or not y.is_contiguous()):
-            raise ValueError(f"graph: y must be a contiguous float64 tensor of {n_out} "
-                             "on the device of x")
-        self.dev, self.x, self.y = dev, x, y
+        order = N.BW_ORDER_ORIGINAL if original else N.BW_ORDER_PERMUTED
         out = C.c_void_p()
-        N.check_hip(N.hip().bw_hip_graph_create(
-            dev._h, C.c_void_p(x.data_ptr()), C.c_void_p(y.data_ptr()),
-            N.BW_ORDER_ORIGINAL if original else N.BW_ORDER_PERMUTED,
-            HipHmat._prec(precision), int(adjoint), C.byref(out)))
+        if batched:
+            X = x
+            if (not isinstance(X, torch.Tensor) or X.dim() != 2 or X.shape[0] != n_in
+                    or X.dtype != torch.float64 or not X.is_cuda or min(X.stride()) < 1):
+                raise ValueError(f"graph_batched: X must be a ({n_in}, k) float64 CUDA tensor "
+                                 "with positive strides")
+            k = X.shape[1]
+            if y is None:
+                y = torch.empty((n_out, k), dtype=torch.float64, device=X.device)
+            if (not isinstance(y, torch.Tensor) or y.shape != (n_out, k)
+                    or y.dtype != torch.float64 or y.device != X.device):
+                raise ValueError(f"graph_batched: Y must be a ({n_out}, {k}) float64 tensor "
+                                 "on the device of X")
+            N.check_hip(N.hip().bw_hip_graph_create_batched(
+                dev._h, int(adjoint), k, C.c_void_p(X.data_ptr()), X.stride(0), X.stride(1),
+                C.c_void_p(y.data_ptr()), y.stride(0), y.stride(1), order,
+                HipHmat._prec(precision), C.byref(out)))
+        else:
+            if not (isinstance(x, torch.Tensor) and x.is_cuda and x.dtype == torch.float64
+                    and x.is_contiguous()):
+                raise TypeError("x must be a contiguous float64 CUDA tensor")
+            if x.numel() != n_in:
+                raise ValueError(f"graph: x has {x.numel()} entries, the operator takes {n_in}")
+            if y is None:
+                y = torch.empty(n_out, dtype=torch.float64, device=x.device)
+            if (not isinstance(y, torch.Tensor) or y.dtype != torch.float64
+                    or y.device != x.device or y.numel() != n_out or not y.is_contiguous()):
+                raise ValueError(f"graph: y must be a contiguous float64 tensor of {n_out} "
+                                 "on the device of x")
+            N.check_hip(N.hip().bw_hip_graph_create(
+                dev._h, C.c_void_p(x.data_ptr()), C.c_void_p(y.data_ptr()), order,
+                HipHmat._prec(precision), int(adjoint), C.byref(out)))
+        self.dev, self.x, self.y = dev, x, y
         self._g = out
 
     def launch(self, stream: int | None = None):

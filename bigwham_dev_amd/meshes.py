@@ -6,6 +6,8 @@ rect_grid      square planar crack of uniform rectangles (3DR0-H, 3DR0-H-mode1),
                CartesianMesh examples (interfaces/python/examples/example_3DR0_kernel.py)
 penny_delaunay penny-shaped crack of radius r: Delaunay triangulation of jittered
                points (3DT0-H)
+crack_family   sides of n square cracks whose element counts are log-uniform
+               in [lo, hi] (many uneven meshes in one operator)
 """
 from __future__ import annotations
 
@@ -68,3 +70,12 @@ def penny_delaunay(n_points: int, radius: float = 1.0, jitter: float = 0.2, seed
     tri[flip] = tri[flip][:, [0, 2, 1]]
     coor = np.concatenate([xy, np.zeros((len(xy), 1))], 1)
     return coor, tri
+
+
+def crack_family(n: int = 16, lo: int = 1000, hi: int = 40000, seed: int = 7):
+    """Sides of n square cracks (rect_grid(side) each) whose element counts
+    are drawn log-uniformly in [lo, hi] with a seeded generator; each side is
+    the rounded square root of its count, at least 4."""
+    rng = np.random.default_rng(seed)
+    counts = np.exp(rng.uniform(np.log(lo), np.log(hi), n))
+    return [max(4, int(round(np.sqrt(c)))) for c in counts]

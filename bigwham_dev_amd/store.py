@@ -116,20 +116,36 @@ class FlatFile:
                 return np.load(path / name, mmap_mode="r", allow_pickle=False)
             except ValueError:  # an empty array cannot be memory-mapped
                 return np.load(path / name, allow_pickle=False)
+        def ld_int(name):
+            a = ld(name)
+            if a.dtype.kind not in "iu":
+                raise ValueError(f"{path}: {name} holds {a.dtype}, not integers")
+            return np.ascontiguousarray(a, dtype=np.int64)
+
         self.meta = meta
         self.dof = d = int(meta["dof_dimension"])
-        self.fr_pattern = np.ascontiguousarray(ld("pattern_FRB.npy"), dtype=np.int64)
-        self.lr_pattern = np.ascontiguousarray(ld("pattern_LRB.npy"), dtype=np.int64)
-        self.perm0 = np.ascontiguousarray(ld("perm_rows.npy"), dtype=np.int64)
-        self.perm1 = np.ascontiguousarray(ld("perm_cols.npy"), dtype=np.int64)
+        self.fr_pattern = ld_int("pattern_FRB.npy")
+        self.lr_pattern = ld_int("pattern_LRB.npy")
+        self.perm0 = ld_int("perm_rows.npy")
+        self.perm1 = ld_int("perm_cols.npy")
         nfr, nlr = int(meta["n_FRB"]), int(meta["n_LRB"])
         if (self.fr_pattern.shape != (nfr, 6) or self.lr_pattern.shape != (nlr, 6)
                 or d * self.perm0.size != meta["m"] or d * self.perm1.size != meta["n"]):
             raise ValueError(f"{path}: pattern or permutation sizes disagree with meta.json")
-        self._fr = ld("frb_data.npy")
-        self._a = ld("lrb_A.npy")
-        self._b = ld("lrb_B.npy")
-        fo, ao, bo = ld("frb_offsets.npy"), ld("lrb_A_offsets.npy"), ld("lrb_B_offsets.npy")
+        def ld_payload(name):
+            # the native code reads 8 bytes per entry through raw pointers:
+            # only 1-D, C-contiguous little-endian float64 is accepted
+            a = ld(name)
+            if a.ndim != 1 or a.dtype != np.dtype("<f8") or not a.flags.c_contiguous:
+                raise ValueError(f"{path}: {name} must be a 1-D little-endian float64 array "
+                                 f"(found {a.dtype}, shape {a.shape})")
+            return a
+
+        self._fr = ld_payload("frb_data.npy")
+        self._a = ld_payload("lrb_A.npy")
+        self._b = ld_payload("lrb_B.npy")
+        fo, ao, bo = (ld_int("frb_offsets.npy"), ld_int("lrb_A_offsets.npy"),
+                      ld_int("lrb_B_offsets.npy"))
         fp, lp = self.fr_pattern, self.lr_pattern
         fr_sz = d * (fp[:, 3] - fp[:, 1]) * d * (fp[:, 4] - fp[:, 2])
         a_sz = d * (lp[:, 3] - lp[:, 1]) * lp[:, 5]

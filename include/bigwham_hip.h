@@ -16,9 +16,17 @@
  * stream when the vectors are device pointers; with host pointers the call
  * copies in, computes and copies out before returning (like
  * HmatCuda::matvec(il::ArrayView), hmat_cuda.cc:2349-2394).  Host threads
- * may call one handle concurrently (its host state is locked), but the GPU
- * work of all calls on a handle shares its scratch vectors: calls issued on
- * different streams must be ordered by the caller.
+ * may call one handle concurrently (its host state is locked).  Each stream
+ * gets its own device scratch (the vector gathered from x, the rank vector
+ * t, partial sums), allocated on the first call from that stream (the null
+ * stream's at creation), so calls on different streams may run at once on
+ * one handle; calls on one stream are ordered by it.  The reference shares
+ * one set of device buffers per object (hmat_cuda.h:131-139).  A stream's
+ * scratch stays allocated until bw_hip_stream_release or bw_hip_destroy;
+ * work on a stream must have completed before that stream is destroyed and
+ * a new stream (which may receive the same handle value) is used.
+ * A caller that captures its own stream into a graph must make one call on
+ * that stream before the capture, so that nothing is allocated during it.
  * Every function returns BW_OK (0) or a negative BW_E* code and never exits
  * the process; bw_hip_last_error() gives the message of the calling thread's
  * last failure.
@@ -117,8 +125,10 @@ int bw_hip_matvec_batched_strided(bw_hip_handle* hd, int adjoint, int nrhs, cons
                                   int64_t y_row_stride, int64_t y_col_stride, int ordering,
                                   int precision, void* stream);
 
-/* Diagonal (length n_rows), from the full-rank blocks that cross the
- * diagonal; ordering as above.  Host pointer. */
+/* Diagonal, from the full-rank blocks that cross the diagonal; ordering as
+ * above.  Host pointer to dof * max(n_row_pts, n_col_pts) doubles: that is
+ * the length Hmat::diagonal returns (hmat.cc:256-285), n_rows for the square
+ * operators BigWham builds; entries past the shorter side are zero. */
 int bw_hip_diagonal(bw_hip_handle* hd, double* d, int ordering);
 
 /* Sizes: rows / cols in DoF, nbOfEntries (algorithmic entries E), device
@@ -128,14 +138,28 @@ int bw_hip_sizes(bw_hip_handle* hd, int64_t* n_rows, int64_t* n_cols,
                  int64_t* stream_bytes_fp64, int64_t* stream_bytes_fp32);
 
 /* A matvec on fixed device vectors x, y captured once as a HIP graph and
- * replayed with one launch (no counterpart in the reference).  The graph uses
- * the handle's scratch, so its launches must not overlap other calls on the
- * handle; destroy it before the handle. */
+ * replayed with one launch (no counterpart in the reference).  The graph owns
+ * its scratch, so its replays may overlap calls on other streams of the
+ * handle; two replays of one graph must be ordered.  Destroy it before the
+ * handle. */
 typedef struct bw_hip_graph bw_hip_graph;
 int bw_hip_graph_create(bw_hip_handle* hd, const double* x, double* y, int ordering,
                         int precision, int adjoint, bw_hip_graph** out);
+/* The batched product of bw_hip_matvec_batched_strided (same arguments and
+ * checks) on fixed device operands X, Y, captured as one graph: every
+ * 32-column round of both passes replays with one launch.  No counterpart in
+ * the reference. */
+int bw_hip_graph_create_batched(bw_hip_handle* hd, int adjoint, int nrhs, const double* X,
+                                int64_t x_row_stride, int64_t x_col_stride, double* Y,
+                                int64_t y_row_stride, int64_t y_col_stride, int ordering,
+                                int precision, bw_hip_graph** out);
 int bw_hip_graph_launch(bw_hip_graph* g, void* stream);
 void bw_hip_graph_destroy(bw_hip_graph* g);
+
+/* Frees the device scratch the handle keeps for `stream` (after waiting for
+ * the stream); the next call from that stream allocates it again.  No
+ * counterpart in the reference. */
+int bw_hip_stream_release(bw_hip_handle* hd, void* stream);
 
 /* Host-only block-diagonal composition (what bw_hip_create_multi lays out),
  * for inspection and CPU checks.  The view borrows the inputs' block
@@ -155,7 +179,9 @@ void bw_flat_composite_free(bw_flat_composite* c);
  * points (original or permuted numbering per `ordering`): the entries whose
  * row and column are both selected, indexed in the selection
  * (HmatSelection::fullBlocksOriginal, hmat_selection.cc:262-371).  Call once
- * with values == NULL to get *count, then with arrays of that length. */
+ * with values == NULL to get *count, then with arrays of that length.  Only
+ * the full-rank blocks are read: h->lr_A / h->lr_B may be NULL (a caller
+ * that released its host low-rank factors after bw_hip_create). */
 int bw_flat_full_blocks(const bw_flat_hmat* h, int ordering, const int64_t* rows,
                         int64_t n_rows, const int64_t* cols, int64_t n_cols, int64_t* count,
                         double* values, int64_t* row_idx, int64_t* col_idx);

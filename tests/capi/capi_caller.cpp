@@ -1,19 +1,27 @@
 // A C++ caller of the engine through its C-ABI only, the way the reference's
 // C++ host code would call it from an Hmat adapter or BigWhamIO::MatVec
 // (src/io/bigwham_io.cc:919-980): build the H-matrix on the host
-// (libbwhost), upload it once (bw_hip_create), then apply it through host
-// pointers, device pointers on a stream, in FP32, transposed and batched, and
-// exercise the error path.  No Python in the loop.
+// (libbwhost), upload it once (bw_hip_create), release the host copies of the
+// low-rank factors as the adapter does, then run the calls the facade makes
+// after the build (BigWhamIO::BuildHierarchicalMatrix -> compressionRatio ->
+// nbOfEntries, bigwham_io.cc:674; GetFullBlocks, bigwham_io.cc:788-905;
+// GetDiagonal) and apply the operator through host pointers, device pointers
+// on a stream, in FP32, transposed and batched, and exercise the error path.
+// No Python in the loop.
 //
 //   capi_caller <kernel> <mesh.bin> <out_dir>
 //
 // mesh.bin: int64 n_nodes, dim, n_elts, n_vertex, then coor (double,
 // n_nodes x dim) and conn (int64, n_elts x n_vertex).  The program writes
 // raw little-endian doubles to out_dir: x, y64, y32, yt, diag, ybatch
-// (n x 3, column-major).  tests/test_gpu_capi.py compares them with the
-// oracle.  Exit status 0 on success; any failed call prints its message.
+// (n x 3, column-major), fb_val (full-block values), and int64 arrays
+// entries (nb_entries, full-block count), fb_row, fb_col.
+// tests/test_gpu_capi.py compares them with the oracle.  Exit status 0 on
+// success; any failed call prints its message.
 #include <hip/hip_runtime_api.h>
 
+#include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -46,10 +54,11 @@ int fail(const char* what, int rc, const char* msg) {
     if (e_ != hipSuccess) return fail(#call, static_cast<int>(e_), hipGetErrorString(e_)); \
   } while (0)
 
-bool write(const std::string& path, const std::vector<double>& v) {
+template <typename T>
+bool write(const std::string& path, const std::vector<T>& v) {
   FILE* f = std::fopen(path.c_str(), "wb");
   if (!f) return false;
-  const bool ok = std::fwrite(v.data(), sizeof(double), v.size(), f) == v.size();
+  const bool ok = std::fwrite(v.data(), sizeof(T), v.size(), f) == v.size();
   return std::fclose(f) == 0 && ok;
 }
 
@@ -80,17 +89,57 @@ int main(int argc, char** argv) {
   bw_flat_hmat flat;
   HOST(bwh_flat(h, &flat));
 
+  // The adapter hands over the reference's own block storage.  Here the
+  // low-rank factors are caller-owned copies, so they can be released after
+  // the upload, as the adapter releases LowRank::A / B.
+  std::vector<std::vector<double>> lr_a(flat.n_lr), lr_b(flat.n_lr);
+  std::vector<const double*> pa(flat.n_lr), pb(flat.n_lr);
+  for (int64_t k = 0; k < flat.n_lr; ++k) {
+    const int64_t* c = flat.lr_pattern + 6 * k;
+    const int64_t r = c[5];
+    const int64_t m = flat.dof * (c[3] - c[1]), nc = flat.dof * (c[4] - c[2]);
+    lr_a[k].assign(flat.lr_A[k], flat.lr_A[k] + m * r);
+    lr_b[k].assign(flat.lr_B[k], flat.lr_B[k] + r * nc);
+    pa[k] = lr_a[k].data();
+    pb[k] = lr_b[k].data();
+  }
+  bw_flat_hmat held = flat;
+  held.lr_A = pa.data();
+  held.lr_B = pb.data();
+
   // one upload, both precisions and the adjoint layout
   bw_hip_handle* hd = nullptr;
-  ENG(bw_hip_create(&flat, BW_PREC_FP64 | BW_PREC_FP32, BW_FLAG_ADJOINT, 0, &hd));
+  ENG(bw_hip_create(&held, BW_PREC_FP64 | BW_PREC_FP32, BW_FLAG_ADJOINT, 0, &hd));
+
+  // release the host low-rank payloads (poisoned first, so any later read of
+  // them would show in the results), keep the full-rank blocks
+  for (int64_t k = 0; k < flat.n_lr; ++k) {
+    std::fill(lr_a[k].begin(), lr_a[k].end(), std::nan(""));
+    std::fill(lr_b[k].begin(), lr_b[k].end(), std::nan(""));
+    std::vector<double>().swap(lr_a[k]);
+    std::vector<double>().swap(lr_b[k]);
+  }
+  held.lr_A = nullptr;
+  held.lr_B = nullptr;
+
+  // compressionRatio() -> nbOfEntries(), as BuildHierarchicalMatrix calls it
   int64_t n_rows, n_cols, entries, dev_bytes, s64, s32;
   ENG(bw_hip_sizes(hd, &n_rows, &n_cols, &entries, &dev_bytes, &s64, &s32));
+  // GetFullBlocks -> fullBlocksOriginal: full-rank entries from the kept blocks
+  int64_t nfb = 0;
+  ENG(bw_flat_full_blocks(&held, BW_ORDER_ORIGINAL, nullptr, 0, nullptr, 0, &nfb, nullptr,
+                          nullptr, nullptr));
+  std::vector<double> fb_val(nfb);
+  std::vector<int64_t> fb_row(nfb), fb_col(nfb);
+  ENG(bw_flat_full_blocks(&held, BW_ORDER_ORIGINAL, nullptr, 0, nullptr, 0, &nfb, fb_val.data(),
+                          fb_row.data(), fb_col.data()));
   const int64_t n = n_rows;
   std::printf("capi_caller: %s n=%lld entries=%lld device bytes=%lld\n", kernel.c_str(),
               static_cast<long long>(n), static_cast<long long>(entries),
               static_cast<long long>(dev_bytes));
 
-  std::vector<double> x(n), y64(n), y32(n), yt(n), yd(n), diag(n);
+  // the diagonal has dof * max(n_row_pts, n_col_pts) entries (n for this square operator)
+  std::vector<double> x(n), y64(n), y32(n), yt(n), yd(n), diag(std::max(n_rows, n_cols));
   uint64_t s = 0x9E3779B97F4A7C15ull;  // x in [-1, 1) from a 64-bit LCG
   for (auto& v : x) {
     s = s * 6364136223846793005ull + 1442695040888963407ull;
@@ -147,9 +196,12 @@ int main(int argc, char** argv) {
   bw_hip_destroy(hd);
   bwh_free(h);
 
+  const std::vector<int64_t> counts = {entries, nfb};
   if (!write(out + "/x.bin", x) || !write(out + "/y64.bin", y64) || !write(out + "/y32.bin", y32) ||
       !write(out + "/yt.bin", yt) || !write(out + "/diag.bin", diag) ||
-      !write(out + "/ybatch.bin", Yb))
+      !write(out + "/ybatch.bin", Yb) || !write(out + "/fb_val.bin", fb_val) ||
+      !write(out + "/fb_row.bin", fb_row) || !write(out + "/fb_col.bin", fb_col) ||
+      !write(out + "/entries.bin", counts))
     return fail("write", -1, out.c_str());
   std::printf("capi_caller: ok\n");
   return 0;

@@ -1,7 +1,9 @@
 """The C-ABI driven from C++ with no Python in the loop
 (tests/capi/capi_caller.cpp, built by __graft_entry__.build()): host build,
-one upload, host- and device-pointer matvecs, FP32, H^T, diagonal, three
-right-hand sides through the batched entry point, and the error codes.
+one upload, the host low-rank factors released (as the Hmat adapter does),
+then the facade's post-build calls (nbOfEntries, full blocks, diagonal),
+host- and device-pointer matvecs, FP32, H^T, three right-hand sides through
+the batched entry point, and the error codes.
 Its outputs are compared here with the oracle on the same H (rebuilt from
 the same mesh by the same host library).  Bars: FP64 1e-12, FP32 1e-5."""
 import subprocess
@@ -54,6 +56,15 @@ def test_cpp_caller(gpu, tmp_path, kernel, mesh):
     assert rel(rd("y32"), y_ref) <= 1e-5
     assert rel(rd("yt"), O.rmatvec_original(f, x)) <= 1e-12
     assert np.array_equal(rd("diag"), O.diagonal(f, original=True))
+    entries, nfb = np.fromfile(tmp_path / "entries.bin", dtype=np.int64)
+    assert entries == O.nb_entries(f) == h.nb_entries()
+    fp = h.fr_pattern()
+    assert nfb == int(np.sum(h.dof * (fp[3] - fp[1]) * h.dof * (fp[4] - fp[2])))
+    from bigwham_dev_amd.hmat import full_blocks
+    vals, ri, ci = full_blocks(h, original=True)
+    assert np.array_equal(rd("fb_val"), vals)
+    assert np.array_equal(np.fromfile(tmp_path / "fb_row.bin", dtype=np.int64), ri)
+    assert np.array_equal(np.fromfile(tmp_path / "fb_col.bin", dtype=np.int64), ci)
     Yb = rd("ybatch").reshape(3, n).T
     for j, c in enumerate((1.0, -1.0, 2.0)):
         assert rel(Yb[:, j], c * y_ref) <= 1e-12, j

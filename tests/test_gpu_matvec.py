@@ -79,6 +79,18 @@ def test_rmatvec(gpu, cases, name):
         assert rel(y32, y_ref) <= TOL32
 
 
+def test_nb_entries(gpu, cases):
+    """nbOfEntries (hmat.cc:299-312) as the device handle counts it, against
+    the oracle's count of the same blocks and the host's own."""
+    for name, h in cases.items():
+        dev = HipHmat(h, precisions=("fp64", "fp32"))
+        print(f"nb_entries {name}: {dev.nb_entries}", flush=True)
+        assert dev.nb_entries == O.nb_entries(h.flat()) == h.nb_entries(), name
+        # the streamed bytes hold every algorithmic entry once, plus padding
+        assert dev.stream_bytes["fp64"] >= 8 * dev.nb_entries
+        assert dev.stream_bytes["fp32"] >= 4 * dev.nb_entries
+
+
 def test_diagonal(gpu, cases):
     for name, h in cases.items():
         dev = HipHmat(h)

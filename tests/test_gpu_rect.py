@@ -118,3 +118,23 @@ def test_rect_composite(gpu):
             assert rel(y[ro[k]:ro[k + 1]], H @ x[co[k]:co[k + 1]]) <= 1e-12
             assert rel(yt[co[k]:co[k + 1]], H.T @ u[ro[k]:ro[k + 1]]) <= 1e-12
     dev.close()
+
+
+def test_rect_diagonal(rect, gpu):
+    """bw_hip_diagonal writes dof * max(n_rows, n_cols) entries (the length
+    Hmat::diagonal returns, hmat.cc:256-285): the FR entries on the diagonal,
+    zero past the shorter side, in both numberings, as the oracle."""
+    S, dev = rect
+    f = S.flat()
+    n = S.dof * max(S.n_row_pts, S.n_col_pts)
+    for original in (False, True):
+        d = dev.diagonal(original=original)
+        assert d.shape == (n,)
+        assert np.array_equal(d, O.diagonal(f, original=original))
+    dp = dev.diagonal(original=False)
+    H = S.dense(False)
+    k = min(H.shape)
+    # entries are either the operator's diagonal (an FR block crosses it) or 0
+    on = dp[:k] != 0
+    assert np.allclose(dp[:k][on], np.diag(H)[on], rtol=0, atol=1e-13 * np.abs(H).max())
+    assert not np.any(dp[k:])

@@ -1,6 +1,7 @@
 """Host threads calling the engine at once (the C-ABI's threading contract):
-several threads on one handle and one stream, and two handles driven from
-two threads on two streams of their own.  Every result must equal the
+several threads on one handle and one stream, two handles driven from two
+threads on two streams of their own, and one handle driven on two streams at
+once (each stream has its own device scratch).  Every result must equal the
 single-threaded one bit for bit."""
 from concurrent.futures import ThreadPoolExecutor
 
@@ -64,3 +65,48 @@ def test_two_handles_two_streams(gpu, host):
         for res in ex.map(work, range(2)):
             for y in res:
                 assert np.array_equal(y, ref)
+
+
+def test_one_handle_two_streams(gpu, host):
+    """One handle, two streams, overlapping on the device: single-vector
+    matvecs in both precisions, H^T and the batched product, each stream with
+    its own inputs.  Results equal the serial run bit for bit."""
+    import torch
+    dev = HipHmat(host, precisions=("fp64", "fp32"), adjoint=True)
+    n = host.size
+    rng = np.random.default_rng(17)
+    xs = [torch.from_numpy(rng.uniform(-1, 1, n)).to(gpu) for _ in range(2)]
+    Xs = [torch.from_numpy(rng.uniform(-1, 1, (n, 32))).to(gpu) for _ in range(2)]
+
+    def calls(i):
+        return (dev.matvec(xs[i]), dev.matvec(xs[i], precision="fp32"), dev.rmatvec(xs[i]),
+                dev.matvec_batched(Xs[i], precision="fp32"), dev.matvec_batched(Xs[i]))
+
+    serial = [[t.cpu().numpy() for t in calls(i)] for i in range(2)]
+    torch.cuda.synchronize()
+    print("one handle, two streams", flush=True)
+    streams = [torch.cuda.Stream(device=gpu) for _ in range(2)]  # default priority
+    for s in streams:
+        s.wait_stream(torch.cuda.default_stream(gpu))
+
+    def work(i):
+        outs = []
+        with torch.cuda.stream(streams[i]):
+            for _ in range(6):  # long enough on the device for the streams to overlap
+                outs.append(calls(i))
+        streams[i].synchronize()
+        return [[t.cpu().numpy() for t in o] for o in outs]
+
+    with ThreadPoolExecutor(2) as ex:
+        res = list(ex.map(work, range(2)))
+    for i in range(2):
+        for o in res[i]:
+            for got, want in zip(o, serial[i]):
+                assert np.array_equal(got, want), i
+    for s in streams:
+        dev.release_stream(s.cuda_stream)
+    # a released stream allocates again on its next call
+    with torch.cuda.stream(streams[0]):
+        y = dev.matvec(xs[0])
+    streams[0].synchronize()
+    assert np.array_equal(y.cpu().numpy(), serial[0][0])

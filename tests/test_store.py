@@ -72,3 +72,24 @@ def test_pattern_only_is_refused(tmp_path):
     h.build_pattern(64, 3.0)
     with pytest.raises(ValueError, match="build it first"):
         store.save(h, tmp_path / "p")
+
+
+@pytest.mark.parametrize("bad", ["float32", "big-endian", "2-D", "int perm", "float offsets"])
+def test_rejects_wrong_dtypes(built, tmp_path, bad):
+    """Payloads the native code would read 8 bytes at a time must be 1-D
+    little-endian float64; permutations and offsets must be integers.  A file
+    with the right element counts but another type is refused, not read."""
+    p = store.save(built, tmp_path / "h")
+    if bad == "float32":
+        np.save(p / "lrb_B.npy", np.load(p / "lrb_B.npy").astype(np.float32))
+    elif bad == "big-endian":
+        np.save(p / "frb_data.npy", np.load(p / "frb_data.npy").astype(">f8"))
+    elif bad == "2-D":
+        a = np.load(p / "lrb_A.npy")
+        np.save(p / "lrb_A.npy", a.reshape(1, -1))
+    elif bad == "int perm":
+        np.save(p / "perm_cols.npy", np.load(p / "perm_cols.npy").astype(np.float64))
+    else:
+        np.save(p / "frb_offsets.npy", np.load(p / "frb_offsets.npy").astype(np.float64))
+    with pytest.raises(ValueError):
+        store.load(p)

@@ -30,6 +30,9 @@ sys.path.insert(0, str(ROOT))
 sys.path.insert(0, str(ROOT / "oracle"))  # cpu_baseline leg only
 
 HBM_PEAK_GBS = 8000.0
+# FP32 copy: low-rank blocks recompressed to this relative accuracy (the FP64
+# copy is the host's blocks unchanged)
+FP32_RANK_TOL = 1e-7
 
 CONFIGS = {
     "c1": dict(kernel="2DS0-H", mesh=("line", 2000), leaf=32, eta=3.0, eps=1e-6,
@@ -99,23 +102,33 @@ def split_entries(host):
 
 def cpu_baseline(host, x, budget_s=15.0):
     """Time the oracle's OpenMP matvec (1 warm-up + up to 10 timed calls,
-    mat_vec.cc:158-168 protocol) within a time budget."""
+    mat_vec.cc:158-168 protocol) within a time budget, built here for this
+    host's CPU (gcc -O3 -march=native -fopenmp).  Returns the result of the
+    default oracle build (the parity reference) and the timing."""
+    import tempfile
     import oracle as O
+    import build_oracle
     flat = host.flat()
     threads = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16")))
-    t0 = time.perf_counter()
+    native = O.native_lib(tempfile.mkdtemp(prefix="bw_oracle_"))
     y = O.matvec_original(flat, x, threads)
+    t0 = time.perf_counter()
+    y_native = O.matvec_original(flat, x, threads, use=native)
     t_warm = time.perf_counter() - t0
     reps = int(max(1, min(10, budget_s // max(t_warm, 1e-6))))
     ts = []
     for _ in range(reps):
         t0 = time.perf_counter()
-        O.matvec_original(flat, x, threads)
+        O.matvec_original(flat, x, threads, use=native)
         ts.append(time.perf_counter() - t0)
     med = float(np.median(ts))
+    agree = float(np.linalg.norm(y_native - y) / np.linalg.norm(y))
+    flags = " ".join(build_oracle.NATIVE_FLAGS)
     return y, dict(value=1.0 / med, unit="H-matvec/s", cores=threads, kind="port",
                    sample=f"{reps} timed calls (+1 warm-up) of the full matvec, median "
-                          f"{med * 1e3:.2f} ms, OpenMP restatement of Hmat::matvecOriginal")
+                          f"{med * 1e3:.2f} ms, OpenMP restatement of Hmat::matvecOriginal "
+                          f"built with gcc {flags} on this host ({threads} threads); "
+                          f"rel. diff to the reference build {agree:.1e}")
 
 
 def dense_check(host, dev, x, max_dense=12000, blocks=8, m=8):
@@ -165,6 +178,9 @@ def main():
     ap.add_argument("--no-gmres", dest="gmres", action="store_false")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--wi-bytes", type=int, default=0)
+    ap.add_argument("--fp32-rank-tol", type=float, default=FP32_RANK_TOL,
+                    help="recompress the FP32 copy's low-rank blocks to this relative "
+                         "accuracy (0: the FP64 factors rounded to FP32)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (calibrates HBM counters)")
     ap.add_argument("--no-autotune", action="store_true",
@@ -208,10 +224,12 @@ def main():
     if args.wi_bytes:
         set_tuning(args.wi_bytes, args.wi_bytes)
     t0 = time.perf_counter()
-    dev = HipHmat(host, precisions=("fp64", "fp32"), device=0)
+    dev = HipHmat(host, precisions=("fp64", "fp32"), device=0, fp32_rank_tol=args.fp32_rank_tol)
     torch.cuda.synchronize()
     t_upload = time.perf_counter() - t0
-    log(f"layout + upload {t_upload:.1f}s, device bytes {dev.device_bytes / 1e9:.2f} GB")
+    ent = dev.entries
+    log(f"layout + upload {t_upload:.1f}s, device bytes {dev.device_bytes / 1e9:.2f} GB, "
+        f"entries fp64 {sum(ent['fp64'][:3])} fp32 {sum(ent['fp32'][:3])}")
     tuning = None
     if not args.no_autotune:
         from bigwham_dev_amd import autotune
@@ -265,15 +283,20 @@ def main():
 
     traffic = load_traffic(args.config)
 
-    def roofline(stats, sm, sv):
-        # algorithmic bytes per launch of each slab kernel
-        b1 = sm * e_b + sv * (n + T)
-        b2 = sm * (e_fr + e_a) + sv * (n + T) + 8 * n  # y is written in float64
+    def roofline(stats, sm, sv, prec):
+        # algorithmic bytes per launch of each slab kernel, from the entries
+        # this precision's copy streams
+        pe_fr, pe_a, pe_b, pT = ent[prec]
+        b1 = sm * pe_b + sv * (n + pT)
+        b2 = sm * (pe_fr + pe_a) + sv * (n + pT) + 8 * n  # y is written in float64
         k1, k2 = stats["p1_slab"], stats["p2_slab"]
         dom, b, ms = ("k_slab pass 2", b2, k2) if k2 >= k1 else ("k_slab pass 1", b1, k1)
-        tr = traffic.get(("fp64" if sm == 8 else "fp32", int(dom[-1])))
+        tr = traffic.get((prec, int(dom[-1])))
+        if prec == "fp32" and args.fp32_rank_tol > 0:
+            tr = None  # the committed PMC figures are of the unrecompressed copy
         ach = b / (ms * 1e-3) / 1e9
-        whole = (sm * E + sv * 4 * n + 2 * 8 * n / host.dof) / (stats["total"] * 1e-3) / 1e9
+        pE = pe_fr + pe_a + pe_b
+        whole = (sm * pE + sv * 4 * n + 2 * 8 * n / host.dof) / (stats["total"] * 1e-3) / 1e9
         return dict(bound="hbm", kernel=dom, achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(ach / HBM_PEAK_GBS, 4), traffic=tr,
                     traffic_source=TRAFFIC_FILES.get(args.config) if tr else None,
@@ -281,8 +304,8 @@ def main():
                     whole_matvec_GBs=round(whole, 1),
                     whole_matvec_frac=round(whole / HBM_PEAK_GBS, 4))
 
-    rf64 = roofline(st64, 8, 8)
-    rf32 = roofline(st32, 4, 4)
+    rf64 = roofline(st64, 8, 8, "fp64")
+    rf32 = roofline(st32, 4, 4, "fp32")
 
     out = {
         "metric": "H-matvec/s",
@@ -302,7 +325,10 @@ def main():
                    "ordering": "original", "inputs": "x ~ U[-1,1] seed 1234, resident in HBM"},
         "roofline": rf64,
         "fp32": {"value": round(args.steps / dt32, 3), "unit": "matvec/s",
-                 "ms_per_step": round(dt32 / args.steps * 1e3, 4), "roofline": rf32},
+                 "ms_per_step": round(dt32 / args.steps * 1e3, 4), "roofline": rf32,
+                 "rank_tolerance": args.fp32_rank_tol,
+                 "nb_entries": int(sum(ent["fp32"][:3])),
+                 "entries_vs_fp64": round(sum(ent["fp32"][:3]) / E, 4)},
         "graph_replay_ms_per_step": graph_ms,
         "setup_s": {"host_build": round(t_build, 2), "layout_upload": round(t_upload, 2)},
         "tuning": tuning,

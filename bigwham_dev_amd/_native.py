@@ -134,6 +134,15 @@ def hip() -> C.CDLL:
                                                     C.c_int64, _vp, C.c_int64, C.c_int64,
                                                     C.c_int, C.c_int, C.POINTER(_vp)]
         lib.bw_hip_stream_release.argtypes = [_vp, _vp]
+        lib.bw_hip_set_fp32_rank_tolerance.argtypes = [C.c_double]
+        lib.bw_hip_entries.argtypes = [_vp, C.c_int, _i64, _i64, _i64, _i64]
+        lib.bw_flat_recompress.argtypes = [C.POINTER(FlatHmat), C.c_double, C.POINTER(_vp)]
+        lib.bw_flat_recompressed_view.argtypes = [_vp]
+        lib.bw_flat_recompressed_view.restype = C.POINTER(FlatHmat)
+        lib.bw_flat_recompressed_stats.argtypes = [_vp, _i64, _i64, _i64]
+        lib.bw_flat_recompress_last_error.restype = C.c_char_p
+        lib.bw_flat_recompressed_free.argtypes = [_vp]
+        lib.bw_flat_recompressed_free.restype = None
         lib.bw_hip_graph_launch.argtypes = [_vp, _vp]
         lib.bw_hip_graph_destroy.argtypes = [_vp]
         lib.bw_hip_graph_destroy.restype = None
@@ -187,6 +196,9 @@ HIP_SYMBOLS = ["bw_hip_create", "bw_hip_matvec", "bw_hip_rmatvec", "bw_hip_matve
                "bw_layout_last_error", "bw_layout_build_multi", "bw_hip_create_multi",
                "bw_hip_graph_create", "bw_hip_graph_launch", "bw_hip_graph_destroy",
                "bw_hip_graph_create_batched", "bw_hip_stream_release",
+               "bw_hip_set_fp32_rank_tolerance", "bw_hip_entries", "bw_flat_recompress",
+               "bw_flat_recompressed_view", "bw_flat_recompressed_stats",
+               "bw_flat_recompress_last_error", "bw_flat_recompressed_free",
                "bw_flat_compose", "bw_flat_composite_view", "bw_flat_composite_offsets",
                "bw_flat_compose_last_error", "bw_flat_composite_free",
                "bw_hip_create_selection", "bw_flat_full_blocks",

@@ -21,9 +21,14 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
 HOST_SOURCES = sorted((CSRC / "host").glob("hb_*.cpp"))
-HIP_SOURCES = [CSRC / "hip" / "engine.hip", CSRC / "hip" / "layout.cpp",
-               CSRC / "hip" / "layout_capi.cpp", CSRC / "hip" / "composite.cpp",
-               CSRC / "hip" / "fullblocks.cpp"]
+# libbwhip.so: the kernels and their C-ABI (hipcc), plus host-only C++ (the
+# slab layout, composition, full blocks, recompression) compiled by g++ with
+# the host ISA of libbwhost.so
+HIP_SOURCES = [CSRC / "hip" / "engine.hip"]
+HIP_HOST_SOURCES = [CSRC / "hip" / "layout.cpp", CSRC / "hip" / "layout_capi.cpp",
+                    CSRC / "hip" / "composite.cpp", CSRC / "hip" / "fullblocks.cpp",
+                    CSRC / "hip" / "recompress.cpp"]
+HOST_FLAGS = ["-std=c++17", "-O3", "-march=x86-64-v3", "-fPIC", "-Wall", "-Wextra"]
 
 
 def _run(cmd: list[str]) -> None:
@@ -38,21 +43,30 @@ def build_host() -> Path:
     LIB.mkdir(exist_ok=True)
     out = LIB / "libbwhost.so"
     tmp = out.with_suffix(".so.tmp")
-    _run(["g++", "-std=c++17", "-O3", "-march=x86-64-v3", "-fPIC", "-shared",
-          "-Wall", "-Wextra", "-o", tmp, *HOST_SOURCES, "-pthread"])
+    _run(["g++", *HOST_FLAGS, "-shared", "-o", tmp, *HOST_SOURCES, "-pthread"])
     os.replace(tmp, out)
     return out
 
 
 def build_hip() -> Path:
     LIB.mkdir(exist_ok=True)
+    obj_dir = PKG / "_build"
+    obj_dir.mkdir(exist_ok=True)
     out = LIB / "libbwhip.so"
     tmp = out.with_suffix(".so.tmp")
-    srcs = [s for s in HIP_SOURCES if s.exists()]
-    _run([HIPCC, f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
-          "-Wno-unused-value", "-Wno-unused-result",
-          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-          "-o", tmp, *srcs, "-pthread"])
+    objs = []
+    for src in HIP_HOST_SOURCES:
+        obj = obj_dir / (src.stem + ".o")
+        _run(["g++", *HOST_FLAGS, "-c", "-o", obj, src])
+        objs.append(obj)
+    for src in HIP_SOURCES:
+        obj = obj_dir / (src.stem + ".o")
+        _run([HIPCC, f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-c",
+              "-Wno-unused-value", "-Wno-unused-result",
+              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+              "-o", obj, src])
+        objs.append(obj)
+    _run([HIPCC, f"--offload-arch={ARCH}", "-fPIC", "-shared", "-o", tmp, *objs, "-pthread"])
     os.replace(tmp, out)
     return out
 

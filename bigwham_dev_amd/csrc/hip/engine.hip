@@ -27,6 +27,7 @@
 
 #include "../../../include/bigwham_hip.h"
 #include "layout.h"
+#include "recompress.h"
 
 namespace {
 
@@ -932,6 +933,8 @@ int host_pinned_alloc(void** p, size_t bytes, const char* what) {
 std::atomic<int64_t> g_wi_bytes64{0};
 std::atomic<int64_t> g_wi_bytes32{0};
 std::atomic<int> g_stack_rows{64};
+// Relative tolerance of the FP32 copy's low-rank recompression (0: off).
+std::atomic<double> g_fp32_rank_tol{0.0};
 
 // About 32k work items: a few rounds of 8 waves on each of the 1024 SIMDs,
 // capped where a sweep of the 600k-DoF 3DT0 matrix on MI355X stopped gaining
@@ -1081,6 +1084,10 @@ struct bw_hip_handle {
   int dof = 1;
   int64_t n_row_pts = 0, n_col_pts = 0;
   int64_t nb_entries = 0;
+  // algorithmic entries of the factors each precision streams: FR blocks,
+  // left factors A, right factors B, and the total rank T ([fp32])
+  int64_t entries[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+  double fp32_rank_tol = 0.0;
   DevOp op[2][2];  // [adjoint][fp32]
   int64_t* perm0 = nullptr;
   int64_t* perm1 = nullptr;
@@ -1444,6 +1451,26 @@ int bw_hip_set_tuning(int64_t wi64, int64_t wi32) {
   return BW_OK;
 }
 
+int bw_hip_set_fp32_rank_tolerance(double rel_tol) {
+  if (!(rel_tol >= 0.0) || !(rel_tol < 1.0))
+    return fail(BW_EINVAL, "set_fp32_rank_tolerance: tolerance must be in [0, 1)");
+  g_fp32_rank_tol = rel_tol;
+  return BW_OK;
+}
+
+int bw_hip_entries(bw_hip_handle* hd, int precision, int64_t* e_full, int64_t* e_left,
+                   int64_t* e_right, int64_t* total_rank) {
+  if (!hd) return fail(BW_EINVAL, "entries: null handle");
+  if (precision != BW_PREC_FP64 && precision != BW_PREC_FP32)
+    return fail(BW_EINVAL, "entries: precision must be BW_PREC_FP64 or BW_PREC_FP32");
+  const int64_t* e = hd->entries[precision == BW_PREC_FP32 ? 1 : 0];
+  if (e_full) *e_full = e[0];
+  if (e_left) *e_left = e[1];
+  if (e_right) *e_right = e[2];
+  if (total_rank) *total_rank = e[3];
+  return BW_OK;
+}
+
 int bw_hip_set_stacking(int stack_rows) {
   if (stack_rows < 0 || stack_rows > 64) return fail(BW_EINVAL, "set_stacking: rows must be 0-64");
   g_stack_rows = stack_rows;
@@ -1517,6 +1544,30 @@ int create_impl(const bw_flat_hmat* h, long double expected_area, int precision_
   if (hipMemcpy(hd->perm0, h->perm0, h->n_row_pts * sizeof(int64_t), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(hd->perm1, h->perm1, h->n_col_pts * sizeof(int64_t), hipMemcpyHostToDevice) != hipSuccess)
     return cleanup(fail(BW_EHIP, "create: permutation upload"));
+  // the FP32 copy may stream recompressed low-rank factors
+  bwl::RecompressedFlat rf;
+  const bw_flat_hmat* h32 = h;
+  hd->fp32_rank_tol = g_fp32_rank_tol.load();
+  if ((precision_mask & BW_PREC_FP32) && hd->fp32_rank_tol > 0.0) {
+    std::string rmsg;
+    if ((rc = bwl::recompress_flat(h, hd->fp32_rank_tol, &rf, &rmsg)))
+      return cleanup(fail(rc, "create: recompression: " + rmsg));
+    h32 = &rf.flat;
+  }
+  for (int f32 = 0; f32 < 2; ++f32) {
+    const bw_flat_hmat* hp = f32 ? h32 : h;
+    int64_t* e = hd->entries[f32];
+    for (int64_t k = 0; k < hp->n_fr; ++k) {
+      const int64_t* c = hp->fr_pattern + 6 * k;
+      e[0] += d * (c[3] - c[1]) * d * (c[4] - c[2]);
+    }
+    for (int64_t k = 0; k < hp->n_lr; ++k) {
+      const int64_t* c = hp->lr_pattern + 6 * k;
+      e[1] += d * (c[3] - c[1]) * c[5];
+      e[2] += c[5] * d * (c[4] - c[2]);
+      e[3] += c[5];
+    }
+  }
   hipStream_t st;
   if (hipStreamCreate(&st) != hipSuccess) return cleanup(fail(BW_EHIP, "create: hipStreamCreate"));
   for (int adj = 0; adj < 2; ++adj) {
@@ -1529,7 +1580,7 @@ int create_impl(const bw_flat_hmat* h, long double expected_area, int precision_
       }
     }
     if (precision_mask & BW_PREC_FP32) {
-      rc = build_op<float>(hd, h, adj != 0, st);
+      rc = build_op<float>(hd, h32, adj != 0, st);
       if (rc) {
         (void)hipStreamDestroy(st);
         return cleanup(rc);

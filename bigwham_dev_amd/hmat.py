@@ -194,13 +194,22 @@ class HipHmat:
     """Device-resident H-matrix on one MI355X (owned by libbwhip)."""
 
     def __init__(self, host: HostHmat, precisions=("fp64",), adjoint: bool = False,
-                 device: int = -1):
+                 device: int = -1, fp32_rank_tol: float = 0.0):
+        """fp32_rank_tol > 0 recompresses the low-rank blocks of the FP32 copy
+        to that relative accuracy (bw_hip_set_fp32_rank_tolerance); the FP64
+        copy is the host's blocks as they are."""
         mask, flags = self._mask(precisions, adjoint)
         out = C.c_void_p()
         self._host = host  # host arrays may be freed after create; kept for convenience
-        N.check_hip(N.hip().bw_hip_create(C.byref(host.flat()), mask, flags, int(device),
+        lib = N.hip()
+        N.check_hip(lib.bw_hip_set_fp32_rank_tolerance(float(fp32_rank_tol)))
+        try:
+            N.check_hip(lib.bw_hip_create(C.byref(host.flat()), mask, flags, int(device),
                                           C.byref(out)))
+        finally:
+            lib.bw_hip_set_fp32_rank_tolerance(0.0)
         self._finish(out, precisions, adjoint, device)
+        self.fp32_rank_tol = float(fp32_rank_tol)
 
     @classmethod
     def many(cls, hosts, precisions=("fp64",), adjoint: bool = False, device: int = -1):
@@ -257,6 +266,12 @@ class HipHmat:
                                          C.byref(db), C.byref(s64), C.byref(s32)))
         self.n_rows, self.n_cols = r.value, c.value
         self.nb_entries = e.value
+        # per precision: (FR entries, A entries, B entries, total rank)
+        self.entries = {}
+        for name in precisions:
+            v = [C.c_int64() for _ in range(4)]
+            N.check_hip(N.hip().bw_hip_entries(self._h, self._prec(name), *map(C.byref, v)))
+            self.entries[name] = tuple(x.value for x in v)
         self.device_bytes = db.value
         self.stream_bytes = {"fp64": s64.value, "fp32": s32.value}
         self.device = int(device)

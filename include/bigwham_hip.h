@@ -191,6 +191,32 @@ const char* bw_flat_full_blocks_last_error(void);
  * the matrix: about 32k work items, 8 KiB to 384 KiB FP64 / 192 KiB FP32). */
 int bw_hip_set_tuning(int64_t wi_bytes_fp64, int64_t wi_bytes_fp32);
 
+/* Low-rank recompression of the FP32 copy for the next create (0, the
+ * default, keeps the FP64 factors rounded to FP32).  With rel_tol > 0 every
+ * low-rank block A B of the FP32 copy becomes the shortest A' B' with
+ * ||A B - A' B'||_F <= rel_tol ||A B||_F, from the singular values of A B
+ * (ACA's ranks exceed that at the same accuracy), so an FP32 matvec streams
+ * fewer bytes.  The FP64 copy is never changed.  No counterpart in the
+ * reference. */
+int bw_hip_set_fp32_rank_tolerance(double rel_tol);
+
+/* Algorithmic entries each precision's copy streams: full-rank blocks, left
+ * factors A, right factors B, and the total rank; the FP64 sums are
+ * nbOfEntries (hmat.cc:299-312), the FP32 ones differ after recompression. */
+int bw_hip_entries(bw_hip_handle* hd, int precision, int64_t* e_full, int64_t* e_left,
+                   int64_t* e_right, int64_t* total_rank);
+
+/* Host-only: the recompression on its own (what bw_hip_set_fp32_rank_tolerance
+ * applies), for inspection and CPU checks.  The view borrows h's full-rank
+ * blocks, permutations and unchanged factors; h must outlive it. */
+typedef struct bw_flat_recompressed bw_flat_recompressed;
+int bw_flat_recompress(const bw_flat_hmat* h, double rel_tol, bw_flat_recompressed** out);
+const bw_flat_hmat* bw_flat_recompressed_view(const bw_flat_recompressed* r);
+int bw_flat_recompressed_stats(const bw_flat_recompressed* r, int64_t* lr_entries_before,
+                               int64_t* lr_entries_after, int64_t* blocks_changed);
+const char* bw_flat_recompress_last_error(void);
+void bw_flat_recompressed_free(bw_flat_recompressed* r);
+
 /* Pass-1 stacking for the next create: right factors whose row chunks have
  * at most `stack_rows` rows share 64-row slabs with the other factors of the
  * same column cluster (0: every factor keeps its own slabs; default 64). */

@@ -29,6 +29,24 @@ def build_oracle() -> Path:
     return out
 
 
+NATIVE_FLAGS = ["-O3", "-march=native", "-fopenmp"]
+
+
+def build_oracle_native(out_dir) -> Path:
+    """The same oracle built for the CPU it runs on (-O3 -march=native), for
+    the host-core baseline timing; built on the machine that times it."""
+    out_dir = Path(out_dir)
+    out_dir.mkdir(parents=True, exist_ok=True)
+    out = out_dir / "liboracle_native.so"
+    cmd = ["gcc", *NATIVE_FLAGS, "-std=c11", "-fPIC", "-shared", "-o", str(out),
+           str(HERE / "hmat_oracle.c"), "-lm"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("native oracle build failed")
+    return out
+
+
 def build_ref() -> Path | None:
     if not Path("/root/reference").is_dir():
         return None

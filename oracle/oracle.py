@@ -16,6 +16,24 @@ HERE = Path(__file__).resolve().parent
 _lib = None
 
 
+def _bind(path) -> C.CDLL:
+    _lib = C.CDLL(str(path))
+    vp = C.c_void_p
+    d = C.POINTER(C.c_double)
+    _lib.oracle_matvec.argtypes = [vp, d, d, C.c_int]
+    _lib.oracle_matvec_original.argtypes = [vp, d, d, C.c_int]
+    _lib.oracle_rmatvec.argtypes = [vp, d, d]
+    _lib.oracle_diagonal.argtypes = [vp, d]
+    _lib.oracle_diagonal_original.argtypes = [vp, d]
+    _lib.oracle_dense.argtypes = [vp, d]
+    _lib.oracle_nb_entries.argtypes = [vp]
+    _lib.oracle_nb_entries.restype = C.c_int64
+    _lib.oracle_exchange_sort.argtypes = [d, C.c_int, C.POINTER(C.c_int64), C.c_int64,
+                                          C.c_int64, C.c_int]
+    _lib.oracle_exchange_sort.restype = None
+    return _lib
+
+
 def lib() -> C.CDLL:
     global _lib
     if _lib is None:
@@ -25,21 +43,17 @@ def lib() -> C.CDLL:
             sys.path.insert(0, str(HERE))
             import build_oracle
             build_oracle.build_oracle()
-        _lib = C.CDLL(str(path))
-        vp = C.c_void_p
-        d = C.POINTER(C.c_double)
-        _lib.oracle_matvec.argtypes = [vp, d, d, C.c_int]
-        _lib.oracle_matvec_original.argtypes = [vp, d, d, C.c_int]
-        _lib.oracle_rmatvec.argtypes = [vp, d, d]
-        _lib.oracle_diagonal.argtypes = [vp, d]
-        _lib.oracle_diagonal_original.argtypes = [vp, d]
-        _lib.oracle_dense.argtypes = [vp, d]
-        _lib.oracle_nb_entries.argtypes = [vp]
-        _lib.oracle_nb_entries.restype = C.c_int64
-        _lib.oracle_exchange_sort.argtypes = [d, C.c_int, C.POINTER(C.c_int64), C.c_int64,
-                                              C.c_int64, C.c_int]
-        _lib.oracle_exchange_sort.restype = None
+        _lib = _bind(path)
     return _lib
+
+
+def native_lib(out_dir) -> C.CDLL:
+    """The oracle compiled for this host's CPU (-O3 -march=native), loaded
+    from out_dir: the host-core baseline of bench.py."""
+    import sys
+    sys.path.insert(0, str(HERE))
+    import build_oracle
+    return _bind(build_oracle.build_oracle_native(out_dir))
 
 
 def _dp(a):
@@ -61,13 +75,14 @@ def matvec(flat, x: np.ndarray, n_threads: int = 0) -> np.ndarray:
     return y
 
 
-def matvec_original(flat, x: np.ndarray, n_threads: int = 0) -> np.ndarray:
-    """Hmat<T>::matvecOriginal (hmat.cc:572-635)."""
+def matvec_original(flat, x: np.ndarray, n_threads: int = 0, use=None) -> np.ndarray:
+    """Hmat<T>::matvecOriginal (hmat.cc:572-635).  use: another build of the
+    oracle (native_lib) instead of the default one."""
     nr, nc = _sizes(flat)
     x = np.ascontiguousarray(x, dtype=np.float64)
     assert x.shape == (nc,)
     y = np.empty(nr)
-    lib().oracle_matvec_original(C.byref(flat), _dp(x), _dp(y), int(n_threads))
+    (use or lib()).oracle_matvec_original(C.byref(flat), _dp(x), _dp(y), int(n_threads))
     return y
 
 

@@ -64,3 +64,43 @@ def test_pipelined_gmres_matches_host_order(gpu, system):
     assert abs(its_gpu - its_cpu) <= 1
     xg = x_gpu.cpu().numpy()
     assert np.linalg.norm(xg - x_cpu.numpy()) / np.linalg.norm(x_cpu.numpy()) < 1e-8
+
+
+@pytest.mark.parametrize("tol", [1e-7, 1e-6])
+def test_recompressed_fp32_copy(gpu, tol):
+    """The FP32 copy with recompressed low-rank blocks
+    (bw_hip_set_fp32_rank_tolerance): its matvec, in both directions, stays
+    within 1e-5 of the oracle on the original blocks while streaming fewer
+    entries; the FP64 copy is unchanged (1e-12); mixed GMRES on it reaches an
+    FP64-verified 1e-8 (residual from the oracle)."""
+    import torch
+    from bigwham_dev_amd.meshes import penny_delaunay
+    h = HostHmat.from_mesh("3DT0-H", *penny_delaunay(4000, seed=1), [1.0, 0.25])
+    h.build(64, 3.0, 1e-6)
+    f = h.flat()
+    print(f"recompressed fp32 copy n={h.size} tol={tol}", flush=True)
+    dev = HipHmat(h, precisions=("fp64", "fp32"), adjoint=True, fp32_rank_tol=tol)
+    e64, e32 = dev.entries["fp64"], dev.entries["fp32"]
+    assert sum(e64[:3]) == O.nb_entries(f) == dev.nb_entries
+    assert e32[0] == e64[0] and e32[1] + e32[2] < e64[1] + e64[2]
+    assert dev.stream_bytes["fp32"] < 0.5 * dev.stream_bytes["fp64"]
+    rng = np.random.default_rng(9)
+    x = rng.uniform(-1, 1, h.size)
+    xd = torch.from_numpy(x).to(gpu)
+    ref = O.matvec_original(f, x)
+    rel = lambda a, b: np.linalg.norm(a - b) / np.linalg.norm(b)
+    assert rel(dev.matvec(xd).cpu().numpy(), ref) <= 1e-12
+    assert rel(dev.matvec(xd, precision="fp32").cpu().numpy(), ref) <= 1e-5
+    reft = O.rmatvec_original(f, x)
+    assert rel(dev.rmatvec(xd, precision="fp32").cpu().numpy(), reft) <= 1e-5
+    X = torch.from_numpy(rng.uniform(-1, 1, (h.size, 32))).to(gpu)
+    Y = dev.matvec_batched(X, precision="fp32").cpu().numpy()
+    for j in (0, 31):
+        assert rel(Y[:, j], O.matvec_original(f, X[:, j].cpu().numpy())) <= 1e-5
+    b = torch.zeros(h.size, dtype=torch.float64, device=gpu)
+    b[2::3] = 1.0  # unit normal pressure
+    solvers.warm_up(dev, b)
+    xm, im = solvers.solve_mixed(dev, b, tol=1e-8)
+    r = np.linalg.norm(b.cpu().numpy() - O.matvec_original(f, xm.cpu().numpy()))
+    r /= np.linalg.norm(b.cpu().numpy())
+    assert im.converged and r <= 1e-8, r

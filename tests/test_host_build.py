@@ -12,7 +12,9 @@ Pins:
     elastic_square_matrix_generator_test.cc:134-182) and penny crack
     w = 8 (1 - nu^2) / (pi E) sqrt(a^2 - r^2) -> t_n = 1;
   * additivity of the influence integral: a rectangle equals the sum of its
-    two triangles (independent closed forms of 3DR0 and 3DT0);
+    two triangles (one polygon formulation serves both kernels, so this
+    checks the element frames, vertex orientation and the triangle kernel's
+    receiver shift; the closed form itself is pinned in tests/test_kernels.py);
   * block counts of src/core/_test/hierarchical_representation_test.cc:19-93;
   * the exact permutation of the reference clustering, replayed in Python on
     top of the oracle's literal exchange sort, and - when the reference

@@ -216,14 +216,18 @@ int oracle_diagonal(const bw_flat_hmat* h, double* diag) {
   return BW_OK;
 }
 
-/* Hmat<T>::diagonalOriginal: permute back with perm1. */
+/* Hmat<T>::diagonalOriginal: permute back with perm1.  The reference loops
+ * over max(rows, cols) points and so reads permutation_1_ past its end when
+ * the operator has more row points than column points (hmat.cc:240-253);
+ * here the loop stops at the column points, which perm1 covers (the entries
+ * past them are zero in `raw` for such operators anyway). */
 int oracle_diagonal_original(const bw_flat_hmat* h, double* diag) {
   const int64_t d = h->dof;
   const int64_t npt = h->n_row_pts > h->n_col_pts ? h->n_row_pts : h->n_col_pts;
   double* raw = (double*)malloc(sizeof(double) * (size_t)(d * npt));
   oracle_diagonal(h, raw);
   memset(diag, 0, sizeof(double) * (size_t)(d * npt));
-  for (int64_t i = 0; i < npt; ++i)
+  for (int64_t i = 0; i < h->n_col_pts; ++i)
     for (int64_t j = 0; j < d; ++j) diag[d * h->perm1[i] + j] = raw[d * i + j];
   free(raw);
   return BW_OK;

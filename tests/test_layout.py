@@ -207,3 +207,30 @@ def test_rectangular_layout_replay(shape, precision, adjoint):
             y = lay.replay(x)
         assert _rel(o, ref) < 1e-13
         assert _rel(y, ref) < tol
+
+
+@pytest.mark.parametrize("shape", [(150, 90, 3), (70, 260, 2)])
+def test_rectangular_oracle_diagonal(shape):
+    """The oracle's diagonal of an operator between point sets of different
+    sizes: dof * max(rows, cols) entries, each the dense operator's diagonal
+    entry where a full-rank block crosses the diagonal and 0 elsewhere, and
+    the original numbering is the permuted one moved by perm1 (more row than
+    column points used to read perm1 past its end)."""
+    from synthetic_hmat import SyntheticHmat
+    S = SyntheticHmat(*shape, seed=sum(shape))
+    f = S.flat()
+    n = S.dof * max(S.n_row_pts, S.n_col_pts)
+    dp = O.diagonal(f, original=False)
+    do = O.diagonal(f, original=True)
+    assert dp.shape == do.shape == (n,)
+    H = S.dense(False)
+    k = min(H.shape)
+    on = dp[:k] != 0
+    assert on.any()
+    assert np.array_equal(dp[:k][on], np.diag(H)[on])
+    assert not np.any(dp[k:])
+    # original numbering: entry dof * perm1[i] + j holds permuted entry dof * i + j
+    idx = (S.dof * np.asarray(S.perm1)[:, None] + np.arange(S.dof)[None, :]).ravel()
+    expect = np.zeros(n)
+    expect[idx] = dp[:idx.size]
+    assert np.array_equal(do, expect)

@@ -97,6 +97,7 @@ def hip() -> C.CDLL:
                                       C.POINTER(_vp)]
         lib.bw_hip_matvec.argtypes = [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp]
         lib.bw_hip_rmatvec.argtypes = [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp]
+        lib.bw_hip_matvec_noalloc.argtypes = [_vp, _vp, _vp, C.c_int, C.c_int, _vp]
         lib.bw_hip_matvec_batched.argtypes = [_vp, C.c_int, _vp, C.c_int64, _vp, C.c_int64,
                                               C.c_int, C.c_int, _vp]
         lib.bw_hip_rmatvec_batched.argtypes = lib.bw_hip_matvec_batched.argtypes
@@ -186,7 +187,7 @@ HOST_SYMBOLS = ["bwh_create", "bwh_create_gaussian", "bwh_build_pattern", "bwh_b
                 "bwh_flat", "bwh_info", "bwh_collocation_points", "bwh_element_normals",
                 "bwh_rotation_matrices", "bwh_dense_block", "bwh_max_aca_error",
                 "bwh_timings", "bwh_free", "bwh_last_error", "bwh_influence"]
-HIP_SYMBOLS = ["bw_hip_create", "bw_hip_matvec", "bw_hip_rmatvec", "bw_hip_matvec_batched",
+HIP_SYMBOLS = ["bw_hip_create", "bw_hip_matvec", "bw_hip_matvec_noalloc", "bw_hip_rmatvec", "bw_hip_matvec_batched",
                "bw_hip_rmatvec_batched", "bw_hip_matvec_batched_strided",
                "bw_hip_diagonal", "bw_hip_sizes", "bw_hip_set_tuning", "bw_hip_set_stacking", "bw_hip_set_unroll", "bw_hip_set_unroll_pass", "bw_hip_set_mfma_schedule",
                "bw_hip_set_timing",

@@ -29,6 +29,13 @@
 #include "layout.h"
 #include "recompress.h"
 
+// Scratch of bw_hip_matvec_noalloc: x gathered in the factor precision and
+// the rank vector t (xt), then the partial sums of both passes, in one fixed
+// array per GPU (each GPU that loads this library holds its own copy until
+// the process ends).  Host code never passes `scratch` itself to a kernel: its
+// device address on each GPU comes from hipGetSymbolAddress (noalloc_bind).
+static __device__ __align__(256) unsigned char scratch[BW_NOALLOC_SCRATCH_BYTES];
+
 namespace {
 
 thread_local std::string g_err;
@@ -1313,7 +1320,7 @@ __global__ __launch_bounds__(256) void k_zero_ranges(T* __restrict__ xt, int64_t
 template <typename T>
 int launch_matvec(bw_hip_handle* hd, DevOp& op, const OpScratch& s, double* y_full,
                   const double* xd, double* yd, int ordering, hipStream_t st,
-                  const SelDir* sd = nullptr) {
+                  const SelDir* sd = nullptr, bool timed = true) {
   const bool orig = !sd && ordering == BW_ORDER_ORIGINAL;
   const int dof = hd->dof;
   T* xt = static_cast<T*>(s.xt);
@@ -1321,7 +1328,8 @@ int launch_matvec(bw_hip_handle* hd, DevOp& op, const OpScratch& s, double* y_fu
   T* part2 = static_cast<T*>(s.part[1]);
   double* const y_user = yd;
   if (sd) yd = y_full;  // the passes write the full permuted y
-  if (hd->timing) (void)hipEventRecord(hd->ev[0], st);
+  const bool timing = timed && hd->timing;
+  if (timing) (void)hipEventRecord(hd->ev[0], st);
   if (sd) {
     if (sd->n_pairs > 0) {
       const int blocks = static_cast<int>((sd->n_pairs * dof + 255) / 256);
@@ -1334,23 +1342,23 @@ int launch_matvec(bw_hip_handle* hd, DevOp& op, const OpScratch& s, double* y_fu
     hipLaunchKernelGGL(k_gather<T>, dim3(blocks), dim3(256), 0, st, xd, xt, n,
                        orig ? op.perm_in : nullptr, dof);
   }
-  if (hd->timing) (void)hipEventRecord(hd->ev[1], st);
+  if (timing) (void)hipEventRecord(hd->ev[1], st);
   if (op.p1.nwi > 0) {
     launch_slab<T, T, 1>(op.unroll[0], op.p1.blocks, st, op.p1.wis, op.p1.xs, op.p1.gidx,
                          static_cast<const T*>(op.p1.data), xt, xt, part1, nullptr, dof,
                          op.p1.rowmap);
   }
-  if (hd->timing) (void)hipEventRecord(hd->ev[2], st);
+  if (timing) (void)hipEventRecord(hd->ev[2], st);
   if (op.p1.nred > 0)
     hipLaunchKernelGGL((k_reduce<T, T>), dim3(op.p1.nred), dim3(64), 0, st, op.p1.reds,
                        op.p1.nred, part1, xt, nullptr, dof, op.p1.rowmap);
-  if (hd->timing) (void)hipEventRecord(hd->ev[3], st);
+  if (timing) (void)hipEventRecord(hd->ev[3], st);
   if (op.p2.nwi > 0) {
     launch_slab<T, double, 2>(op.unroll[1], op.p2.blocks, st, op.p2.wis, op.p2.xs, op.p2.gidx,
                               static_cast<const T*>(op.p2.data), xt, yd, part2,
                               orig ? op.perm_out : nullptr, dof, nullptr);
   }
-  if (hd->timing) (void)hipEventRecord(hd->ev[4], st);
+  if (timing) (void)hipEventRecord(hd->ev[4], st);
   if (op.p2.nred > 0)
     hipLaunchKernelGGL((k_reduce<T, double>), dim3(op.p2.nred), dim3(64), 0, st, op.p2.reds,
                        op.p2.nred, part2, yd, orig ? op.perm_out : nullptr, dof, nullptr);
@@ -1359,7 +1367,7 @@ int launch_matvec(bw_hip_handle* hd, DevOp& op, const OpScratch& s, double* y_fu
     hipLaunchKernelGGL(k_gather_sel, dim3(blocks), dim3(256), 0, st, yd, y_user, sd->out_dst,
                        sd->n_out, dof);
   }
-  if (hd->timing) (void)hipEventRecord(hd->ev[5], st);
+  if (timing) (void)hipEventRecord(hd->ev[5], st);
   HIPCHK(hipGetLastError());
   return BW_OK;
 }
@@ -1438,9 +1446,104 @@ int do_matvec(bw_hip_handle* hd, bool adjoint, const double* x, double* y, int o
   return BW_OK;
 }
 
+// Device address of `scratch` on each GPU (index: device ordinal), taken
+// once per device by noalloc_bind with that device current; nullptr until
+// then, or if hipGetSymbolAddress failed there (noalloc_err holds why).
+std::mutex g_noalloc_mu;
+std::vector<void*> g_noalloc_ptr;
+std::vector<std::string> g_noalloc_err;
+
+// Called by every create on its device (current).  A failure only disables
+// bw_hip_matvec_noalloc on that device; the other entry points never use
+// the array.
+void noalloc_bind(int device) {
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  if (static_cast<int>(g_noalloc_ptr.size()) <= device) {
+    g_noalloc_ptr.resize(device + 1, nullptr);
+    g_noalloc_err.resize(device + 1);
+  }
+  if (g_noalloc_ptr[device]) return;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    g_noalloc_err[device] = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+    return;
+  }
+  g_noalloc_ptr[device] = p;
+}
+
+int noalloc_matvec(bw_hip_handle* hd, const double* x, double* y, int ordering, int precision,
+                   void* stream) {
+  if (!hd || !x || !y) return fail(BW_EINVAL, "matvec_noalloc: null argument");
+  if (ordering != BW_ORDER_PERMUTED && ordering != BW_ORDER_ORIGINAL)
+    return fail(BW_EINVAL, "matvec_noalloc: ordering must be 0 (permuted) or 1 (original)");
+  if (precision != BW_PREC_FP64 && precision != BW_PREC_FP32)
+    return fail(BW_EINVAL, "matvec_noalloc: precision must be BW_PREC_FP64 or BW_PREC_FP32");
+  if (hd->sel.on) return fail(BW_ENOTSUP, "matvec_noalloc: not available on a selection");
+  const int f32 = precision == BW_PREC_FP32 ? 1 : 0;
+  DevOp& op = hd->op[0][f32];
+  if (!op.present) return fail(BW_ENOTSUP, "matvec_noalloc: precision not built for this handle");
+  // xt, then the partial sums of pass 1 and pass 2, each 256-byte aligned
+  const auto up = [](int64_t b) { return (b + 255) / 256 * 256; };
+  const int64_t off1 = up(op.xt_size * op.elem);
+  const int64_t off2 = off1 + up(op.part_elems[0] * op.elem);
+  const int64_t need = off2 + up(op.part_elems[1] * op.elem);
+  int64_t limit = BW_NOALLOC_SCRATCH_BYTES;
+  if (const char* e = std::getenv("BWHIP_NOALLOC_LIMIT")) {
+    const long long v = std::atoll(e);
+    if (v > 0 && v < limit) limit = v;  // tests of the size check may only lower it
+  }
+  if (need > limit)
+    return fail(BW_ESHAPE, "matvec_noalloc: this operator needs " + std::to_string(need) +
+                               " bytes of scratch; the fixed array holds " + std::to_string(limit) +
+                               " (use bw_hip_matvec)");
+  void* base = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    if (hd->device < static_cast<int>(g_noalloc_ptr.size())) base = g_noalloc_ptr[hd->device];
+    if (!base)
+      return fail(BW_EHIP, "matvec_noalloc: no scratch address on this device: " +
+                               (hd->device < static_cast<int>(g_noalloc_err.size())
+                                    ? g_noalloc_err[hd->device]
+                                    : std::string("not bound")));
+  }
+  std::lock_guard<std::mutex> lock(hd->mu);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  HIPCHK(hipSetDevice(hd->device));
+  unsigned char* b = static_cast<unsigned char*>(base);
+  OpScratch s;
+  s.xt = b;
+  s.part[0] = b + off1;
+  s.part[1] = b + off2;
+  // Padded column groups read xt past x (the gap) and past t (the tail), so
+  // those must hold zeros; an earlier call on this GPU may have left other
+  // values there.  x and t are rewritten by every call, and every partial
+  // sum is written before it is read.  A fill kernel, not hipMemsetAsync,
+  // so the call can be captured into a graph.
+  const int64_t na = op.xt_t - op.n_in, nb = op.xt_size - (op.xt_t + op.t_total);
+  const int zblocks = static_cast<int>((na + nb + 255) / 256);
+  if (zblocks > 0) {
+    if (f32)
+      hipLaunchKernelGGL(k_zero_ranges<float>, dim3(zblocks), dim3(256), 0, st,
+                         static_cast<float*>(s.xt), op.n_in, op.xt_t, op.xt_t + op.t_total,
+                         op.xt_size);
+    else
+      hipLaunchKernelGGL(k_zero_ranges<double>, dim3(zblocks), dim3(256), 0, st,
+                         static_cast<double*>(s.xt), op.n_in, op.xt_t, op.xt_t + op.t_total,
+                         op.xt_size);
+  }
+  return f32 ? launch_matvec<float>(hd, op, s, nullptr, x, y, ordering, st, nullptr, false)
+             : launch_matvec<double>(hd, op, s, nullptr, x, y, ordering, st, nullptr, false);
+}
+
 }  // namespace
 
 extern "C" {
+
+int bw_hip_matvec_noalloc(bw_hip_handle* hd, const double* x, double* y, int ordering,
+                          int precision, void* stream) {
+  return noalloc_matvec(hd, x, y, ordering, precision, stream);
+}
 
 const char* bw_hip_last_error(void) { return g_err.c_str(); }
 
@@ -1533,6 +1636,7 @@ int create_impl(const bw_flat_hmat* h, long double expected_area, int precision_
     return code;
   };
   if (hipSetDevice(hd->device) != hipSuccess) return cleanup(fail(BW_EHIP, "create: hipSetDevice"));
+  noalloc_bind(hd->device);  // once per device: the address of bw_hip_matvec_noalloc's array
   hd->dof = h->dof;
   hd->n_row_pts = h->n_row_pts;
   hd->n_col_pts = h->n_col_pts;

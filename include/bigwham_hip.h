@@ -95,6 +95,23 @@ int bw_hip_create_selection(const bw_flat_hmat* h, const int64_t* rows, int64_t 
 int bw_hip_matvec(bw_hip_handle* hd, const double* x, double* y, int ordering,
                   int precision, int on_device, void* stream);
 
+/* bw_hip_matvec on device pointers that allocates nothing: no counterpart in
+ * the reference.  It is for callers that cannot allocate device memory:
+ * code under HIP graph capture, tight serving loops, and harnesses that
+ * forbid hipMalloc.  Its scratch (x gathered in the factor precision, the
+ * rank vector t, the partial sums of both passes) lives in one fixed device
+ * array of BW_NOALLOC_SCRATCH_BYTES per GPU, which every GPU that loads the
+ * library holds until the process ends.  An operator that needs more
+ * returns BW_ESHAPE with the size it needs; bw_hip_matvec serves it instead.
+ * Every call on a GPU uses that GPU's one array, whatever the handle or
+ * stream: calls that overlap on one GPU race, so the caller orders them
+ * (one stream, or events).  Each call clears what must start at zero.
+ * x and y are device pointers; selections return BW_ENOTSUP; the per-kernel
+ * timers do not record these calls. */
+#define BW_NOALLOC_SCRATCH_BYTES (64LL << 20)
+int bw_hip_matvec_noalloc(bw_hip_handle* hd, const double* x, double* y, int ordering,
+                          int precision, void* stream);
+
 /* y = H^T x (requires BW_FLAG_ADJOINT at creation). */
 int bw_hip_rmatvec(bw_hip_handle* hd, const double* x, double* y, int ordering,
                    int precision, int on_device, void* stream);

@@ -150,21 +150,62 @@ __global__ __launch_bounds__(64) void k_slab(const bwl::WorkItem* __restrict__ w
     if (lane < R) {
       const VT* p = base + lane;
       int g = 0;
-      for (; g + U <= ng; g += U) {
-        VT a[U];
+      // Full batches of U groups through two register buffers: the loads of
+      // batch b + 1 go out before the FMAs of batch b, so a batch's loads are
+      // in flight while the previous one is consumed (one buffer drained to
+      // nothing before the next loads went out).  The loop body issues the
+      // same loads on every trip and the last one or two batches are peeled,
+      // so the waits the compiler places are the same on every path.  Same
+      // groups, same accumulator per group and same order: same bits.
+      const int nfull = ng / U;
+      if (nfull > 0) {
+        VT a[U], b[U];
+        int za[U], zb[U];
+        auto load = [&](int g0, VT (&buf)[U], int (&zz)[U]) {
 #pragma unroll
-        for (int u = 0; u < U; ++u) a[u] = __builtin_nontemporal_load(p + (g + u) * R);
+          for (int u = 0; u < U; ++u) zz[u] = gi[g0 + u];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-          const int z = __builtin_amdgcn_readfirstlane(gi[g + u]);
+          for (int u = 0; u < U; ++u) buf[u] = __builtin_nontemporal_load(p + (g0 + u) * R);
+        };
+        auto use = [&](const VT (&buf)[U], const int (&zz)[U]) {
 #pragma unroll
-          for (int v = 0; v < V; ++v) {
-            if (u & 1)
-              acc1 += a[u][v] * xin[z + v];
-            else
-              acc0 += a[u][v] * xin[z + v];
+          for (int u = 0; u < U; ++u) {
+            // unsigned: no sign extension of the loaded offset is placed
+            // (and waited for) next to its load
+            const uint32_t z = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(zz[u]));
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+              if (u & 1)
+                acc1 += buf[u][v] * xin[z + v];
+              else
+                acc0 += buf[u][v] * xin[z + v];
+            }
           }
+        };
+        load(0, a, za);
+        int bt = 0;
+        for (; bt + 2 < nfull; bt += 2) {
+          __builtin_amdgcn_sched_barrier(0);
+          load((bt + 1) * U, b, zb);
+          __builtin_amdgcn_sched_barrier(0);
+          use(a, za);
+          __builtin_amdgcn_sched_barrier(0);
+          load((bt + 2) * U, a, za);
+          __builtin_amdgcn_sched_barrier(0);
+          use(b, zb);
         }
+        __builtin_amdgcn_sched_barrier(0);
+        if (bt + 2 == nfull) {
+          load((bt + 1) * U, b, zb);
+          __builtin_amdgcn_sched_barrier(0);
+          use(a, za);
+          __builtin_amdgcn_sched_barrier(0);
+          use(b, zb);
+        } else {
+          use(a, za);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        g = nfull * U;
       }
       // the last, partial batch in steps of four groups, then one at a time
       for (; g + 4 <= ng; g += 4) {
@@ -519,7 +560,84 @@ __device__ __forceinline__ void mfma_stream_deep(const flt4* __restrict__ base,
   }
 }
 
-template <int PASS, int KU, bool PIPE, int D = 0>
+// Ring variant with the loads issued in the order they are consumed: at
+// stage j the MFMAs of buffer j run (its loads went out one trip, D stages,
+// earlier), then buffer j is refilled with the step D ahead and the input
+// offsets of the step after that.  The prologue fills the buffers in the same
+// order as the loop, so every stage has the same D - 1 younger steps in
+// flight on every path (the loop above refills a buffer before the MFMAs and
+// its prologue order differs from the loop's, so the wait at the top of the
+// loop covered only one step).  Loads and MFMAs are unconditional, so that
+// the loop body is one basic block and its waits are the same on every
+// path: past the last step the loads read clamped, in-bounds addresses and
+// the MFMAs get B = 0 (as groups past the end do), adding nothing.
+template <int KU, int D, bool TWO>
+__device__ __forceinline__ void mfma_stream_ring(const flt4* __restrict__ base,
+                                                 const int32_t* __restrict__ gi,
+                                                 const float* __restrict__ XB, int ng, int R,
+                                                 int half, int r0, floatx16& acc0,
+                                                 floatx16& acc1) {
+  constexpr int G = 2 * KU;  // groups per step
+  const int rA0 = min(r0, R - 1), rA1 = min(r0 + 32, R - 1);
+  const int nsteps = (ng + G - 1) / G;
+  flt4 a0[D][KU], a1[D][KU];
+  float b[D][KU][4];
+  int zz[D][KU];  // input offsets of the step buffer k is refilled with next
+  auto gz = [&](int step, int u) { return gi[min(step * G + 2 * u + half, ng - 1)]; };
+  auto load = [&](int step, const int (&z)[KU], flt4 (&x0)[KU], flt4 (&x1)[KU],
+                  float (&bb)[KU][4]) {
+#pragma unroll
+    for (int u = 0; u < KU; ++u) {
+      const int ga = min(step * G + 2 * u + half, ng - 1);
+      x0[u] = __builtin_nontemporal_load(base + ga * R + rA0);
+      if (TWO) x1[u] = __builtin_nontemporal_load(base + ga * R + rA1);
+      // 32-bit offsets (xt_size * kRhs < 2^32, checked by the batched calls):
+      // no sign extension of the loaded offset sits between its load and here
+      const uint32_t zo = static_cast<uint32_t>(z[u]) * kRhs + r0;
+#pragma unroll
+      for (int s = 0; s < 4; ++s) bb[u][s] = XB[zo + s * kRhs];
+    }
+  };
+  auto compute = [&](int step, const flt4 (&x0)[KU], const flt4 (&x1)[KU],
+                     const float (&bb)[KU][4]) {
+#pragma unroll
+    for (int u = 0; u < KU; ++u) {
+      const bool ok = step * G + 2 * u + half < ng;
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const float bv = ok ? bb[u][s] : 0.0f;
+        acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(x0[u][s], bv, acc0, 0, 0, 0);
+        if (TWO) acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(x1[u][s], bv, acc1, 0, 0, 0);
+      }
+    }
+  };
+#pragma unroll
+  for (int k = 0; k < D; ++k)
+#pragma unroll
+    for (int u = 0; u < KU; ++u) zz[k][u] = gz(k, u);
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    __builtin_amdgcn_sched_barrier(0);
+    load(k, zz[k], a0[k], a1[k], b[k]);
+#pragma unroll
+    for (int u = 0; u < KU; ++u) zz[k][u] = gz(k + D, u);
+  }
+  for (int s0 = 0; s0 < nsteps; s0 += D) {
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      const int s = s0 + j;
+      __builtin_amdgcn_sched_barrier(0);
+      compute(s, a0[j], a1[j], b[j]);  // past the last step: B = 0, adds nothing
+      __builtin_amdgcn_sched_barrier(0);
+      load(s + D, zz[j], a0[j], a1[j], b[j]);
+#pragma unroll
+      for (int u = 0; u < KU; ++u) zz[j][u] = gz(s + 2 * D, u);
+    }
+  }
+  __builtin_amdgcn_sched_barrier(0);
+}
+
+template <int PASS, int KU, bool PIPE, int D = 0, bool RING = false>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_slab_mfma(
     const bwl::WorkItem* __restrict__ wis, XcdSplit xs, const int32_t* __restrict__ gidx,
     const float* __restrict__ data, const float* __restrict__ XB,
@@ -540,7 +658,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
   floatx16 acc0 = {0}, acc1 = {0};
   const bool two = R > 32;
   if (ng > 0) {
-    if constexpr (D > 0) {
+    if constexpr (RING) {
+      if (two)
+        mfma_stream_ring<KU, D, true>(base, gi, XB, ng, R, half, r0, acc0, acc1);
+      else
+        mfma_stream_ring<KU, D, false>(base, gi, XB, ng, R, half, r0, acc0, acc1);
+    } else if constexpr (D > 0) {
       if (two)
         mfma_stream_deep<KU, D, true>(base, gi, XB, ng, R, half, r0, acc0, acc1);
       else
@@ -586,6 +709,148 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
   }
 }
 
+// The same product on v_mfma_f32_16x16x4f32 with 16-row tiles, so a slab of
+// R rows costs ceil(R / 16) tiles instead of 32 or 64 rows (pass-1 slabs of
+// 33-48 rows carry a third of their MFMA work as padding on 32-row tiles).
+// One k-step covers four column groups: lane l supplies A[i][k] with
+// i = l % 16 (row within the tile) and k = l / 16, i.e. element s of row i of
+// group 4 step + k is MFMA s's operand, and B[k][n] = XB[z(group) + s][n]
+// for n = l % 16 (+ 16 for the second 16 right-hand sides).  The same
+// 16-byte slab vectors are read as in the 32 x 32 kernel.  Loads run through
+// a ring of D step buffers in load order, as mfma_stream_ring; every tile's
+// rows are loaded (clamped, in bounds) and tiles past ceil(R / 16) skip their
+// MFMAs.  Accumulator element r of lane l is row 4 (l / 16) + r, column
+// l % 16 of its tile.
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+// The k-steps of k_slab_mfma16 for a slab of NT 16-row tiles (NT a template
+// argument, so the loop body is one basic block with the same waits on
+// every path).
+template <int NT, int D>
+__device__ __forceinline__ void mfma16_stream(const flt4* __restrict__ base,
+                                              const int32_t* __restrict__ gi,
+                                              const float* __restrict__ XB, int ng, int R, int kq,
+                                              int i16, floatx4 (&acc)[4][2]) {
+  int rowc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) rowc[t] = min(16 * t + i16, R - 1);
+  const int nsteps = (ng + 3) >> 2;  // four groups per k-step
+  flt4 a[D][NT];
+  float b[D][2][4];
+  int zz[D];  // input offset of the step buffer q is refilled with next
+  auto gz = [&](int step) { return gi[min(4 * step + kq, ng - 1)]; };
+  auto load = [&](int step, int z, flt4 (&aa)[NT], float (&bb)[2][4]) {
+    const int ga = min(4 * step + kq, ng - 1);
+#pragma unroll
+    for (int t = 0; t < NT; ++t) aa[t] = __builtin_nontemporal_load(base + ga * R + rowc[t]);
+    const uint32_t zo = static_cast<uint32_t>(z) * kRhs + i16;  // < 2^32, checked
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      bb[0][s] = XB[zo + s * kRhs];
+      bb[1][s] = XB[zo + s * kRhs + 16];
+    }
+  };
+  auto compute = [&](int step, const flt4 (&aa)[NT], const float (&bb)[2][4]) {
+    const bool ok = 4 * step + kq < ng;
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const float b0 = ok ? bb[0][s] : 0.0f, b1 = ok ? bb[1][s] : 0.0f;
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        acc[t][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(aa[t][s], b0, acc[t][0], 0, 0, 0);
+        acc[t][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(aa[t][s], b1, acc[t][1], 0, 0, 0);
+      }
+    }
+  };
+#pragma unroll
+  for (int q = 0; q < D; ++q) zz[q] = gz(q);
+#pragma unroll
+  for (int q = 0; q < D; ++q) {
+    __builtin_amdgcn_sched_barrier(0);
+    load(q, zz[q], a[q], b[q]);
+    zz[q] = gz(q + D);
+  }
+  for (int s0 = 0; s0 < nsteps; s0 += D) {
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      const int st = s0 + j;
+      __builtin_amdgcn_sched_barrier(0);
+      compute(st, a[j], b[j]);  // past the last step: B = 0, adds nothing
+      __builtin_amdgcn_sched_barrier(0);
+      load(st + D, zz[j], a[j], b[j]);
+      zz[j] = gz(st + 2 * D);
+    }
+  }
+  __builtin_amdgcn_sched_barrier(0);
+}
+
+template <int PASS, int D>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(D > 3 ? 3 : 4))) void k_slab_mfma16(
+    const bwl::WorkItem* __restrict__ wis, XcdSplit xs, const int32_t* __restrict__ gidx,
+    const float* __restrict__ data, const float* __restrict__ XB,
+    float* __restrict__ tout,              // pass 1
+    double* __restrict__ Y, int64_t yr, int64_t yc,  // pass 2 (row, column strides)
+    int nrhs, float* __restrict__ part, const int64_t* __restrict__ perm, int dof,
+    const int32_t* __restrict__ rowmap) {
+  const int lane = threadIdx.x & 63;
+  const int w = xcd_work_item<1>(xs);
+  if (w < 0) return;
+  const bwl::WorkItem wi = wis[w];
+  const int R = wi.rows;
+  const int ng = wi.ngroups;
+  const int kq = lane >> 4, i16 = lane & 15;
+  const flt4* __restrict__ base = reinterpret_cast<const flt4*>(data + wi.data_off);
+  const int32_t* __restrict__ gi = gidx + wi.gbeg;
+  const int ntile = (R + 15) >> 4;  // wave-uniform
+  floatx4 acc[4][2];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) acc[t][0] = acc[t][1] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+  if (ng > 0) {
+    switch (ntile) {
+      case 1: mfma16_stream<1, D>(base, gi, XB, ng, R, kq, i16, acc); break;
+      case 2: mfma16_stream<2, D>(base, gi, XB, ng, R, kq, i16, acc); break;
+      case 3: mfma16_stream<3, D>(base, gi, XB, ng, R, kq, i16, acc); break;
+      default: mfma16_stream<4, D>(base, gi, XB, ng, R, kq, i16, acc);
+    }
+  }
+  if (wi.part_off >= 0) {
+    float* pp = part + static_cast<int64_t>(wi.part_off) * kRhs;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      if (t >= ntile) break;
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = 16 * t + 4 * kq + r;
+          if (row < R) pp[row * kRhs + 16 * h + i16] = acc[t][h][r];  // added in order by k_reduce_b
+        }
+    }
+    return;
+  }
+  // two tiles (8 of this lane's rows) at a time: their output rows first
+  // (clamped, in bounds, no branch), so the index loads go out together
+#pragma unroll
+  for (int t0 = 0; t0 < 4; t0 += 2) {
+    if (t0 >= ntile) break;
+    int idx[8], orow[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) idx[j] = wi.out_row + min(16 * (t0 + (j >> 2)) + 4 * kq + (j & 3), R - 1);
+    batched_rows<PASS>(idx, rowmap, perm, dof, orow);
+#pragma unroll
+    for (int t = t0; t < t0 + 2; ++t)
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = 16 * t + 4 * kq + r;
+          const int n = 16 * h + i16;
+          if (row < R && n < nrhs)
+            batched_put<PASS>(tout, Y, yr, yc, orow[4 * (t - t0) + r], n, acc[t][h][r]);
+        }
+  }
+}
+
 template <int PASS>
 void launch_mfma(int schedule, int blocks, hipStream_t st, const bwl::WorkItem* wis, XcdSplit xs,
                  const int32_t* gidx, const float* data, const float* XB, float* tout, double* Y,
@@ -619,6 +884,22 @@ void launch_mfma(int schedule, int blocks, hipStream_t st, const bwl::WorkItem* 
     case 7:
       hipLaunchKernelGGL((k_slab_mfma<PASS, 1, true, 8>), dim3(4 * blocks), dim3(64), 0, st, wis, xs,
                          gidx, data, XB, tout, Y, yr, yc, nrhs, part, perm, dof, rowmap);
+      break;
+#define BW_MFMA32_RING(KU, D)                                                              \
+  hipLaunchKernelGGL((k_slab_mfma<PASS, KU, true, D, true>), dim3(4 * blocks), dim3(64), 0, st, \
+                     wis, xs, gidx, data, XB, tout, Y, yr, yc, nrhs, part, perm, dof, rowmap)
+    case 8: BW_MFMA32_RING(2, 3); break;
+    case 9: BW_MFMA32_RING(2, 4); break;
+    case 10: BW_MFMA32_RING(1, 6); break;
+    case 11: BW_MFMA32_RING(1, 8); break;
+#undef BW_MFMA32_RING
+    case 12:
+      hipLaunchKernelGGL((k_slab_mfma16<PASS, 3>), dim3(4 * blocks), dim3(64), 0, st, wis, xs, gidx,
+                         data, XB, tout, Y, yr, yc, nrhs, part, perm, dof, rowmap);
+      break;
+    case 13:
+      hipLaunchKernelGGL((k_slab_mfma16<PASS, 4>), dim3(4 * blocks), dim3(64), 0, st, wis, xs, gidx,
+                         data, XB, tout, Y, yr, yc, nrhs, part, perm, dof, rowmap);
       break;
     default:
       hipLaunchKernelGGL((k_slab_mfma<PASS, 4, false>), dim3(4 * blocks), dim3(64), 0, st, wis, xs,
@@ -687,7 +968,7 @@ typedef double dblx4 __attribute__((ext_vector_type(4)));
 // mfma_stream_deep), D - 1 steps of loads in flight; D = 0: two steps loaded,
 // then their MFMAs.  Both issue the same MFMAs in the same order per
 // accumulator, so the results agree bit for bit.
-template <int PASS, int D>
+template <int PASS, int D, bool RING = false>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(D > 4 ? 2 : (D > 0 ? 3 : 4)))) void k_slab_mfma64(
     const bwl::WorkItem* __restrict__ wis, XcdSplit xs, const int32_t* __restrict__ gidx,
     const double* __restrict__ data, const double* __restrict__ XB,
@@ -721,10 +1002,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(D > 4 ? 2 : 
       const int ga = min(2 * step + gsub, ng - 1);
       const double* __restrict__ gp = base + static_cast<int64_t>(ga) * R * 2 + v;
 #pragma unroll
-      for (int t = 0; t < 4; ++t) aa[t] = t < ntile ? gp[rowc[t] * 2] : 0.0;
-      const int64_t zr = static_cast<int64_t>(z + v) * kRhs;
-      bb[0] = XB[zr + i16];
-      bb[1] = XB[zr + 16 + i16];
+      // every tile loads (rows clamped into the slab, in bounds) so that the
+      // loads, and hence the waits, are the same on every path; tiles past
+      // ntile are never multiplied
+      for (int t = 0; t < 4; ++t) aa[t] = gp[rowc[t] * 2];
+      const uint32_t zr = static_cast<uint32_t>(z + v) * kRhs + i16;  // < 2^32, checked
+      bb[0] = XB[zr];
+      bb[1] = XB[zr + 16];
     };
     auto compute = [&](int step, const double (&aa)[4], const double (&bb)[2]) {
       const bool ok = 2 * step + gsub < ng;
@@ -738,6 +1022,28 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(D > 4 ? 2 : 
     };
 #pragma unroll
     for (int q = 0; q < D; ++q) zz[q] = gz(q);
+    if constexpr (RING) {
+      // buffers consumed in the order they were loaded (as mfma_stream_ring):
+      // compute buffer j, then refill it with the step D ahead
+#pragma unroll
+      for (int q = 0; q < D; ++q) {
+        __builtin_amdgcn_sched_barrier(0);
+        load(q, zz[q], a[q], b[q]);
+        zz[q] = gz(q + D);
+      }
+      for (int s0 = 0; s0 < nsteps; s0 += D) {
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+          const int st = s0 + j;
+          __builtin_amdgcn_sched_barrier(0);
+          compute(st, a[j], b[j]);  // past the last step: B = 0, adds nothing
+          __builtin_amdgcn_sched_barrier(0);
+          load(st + D, zz[j], a[j], b[j]);
+          zz[j] = gz(st + 2 * D);
+        }
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    } else {
 #pragma unroll
     for (int q = 0; q < D - 1; ++q) {
       load(q, zz[q], a[q], b[q]);
@@ -754,6 +1060,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(D > 4 ? 2 : 
         if (st < nsteps) compute(st, a[j], b[j]);
         __builtin_amdgcn_sched_barrier(0);
       }
+    }
     }
   } else {
   constexpr int S = 2;  // k-steps (two groups each) loaded before their MFMAs
@@ -829,14 +1136,24 @@ void launch_mfma64(int schedule, int blocks, hipStream_t st, const bwl::WorkItem
 #define BW_MFMA64(D)                                                                        \
   hipLaunchKernelGGL((k_slab_mfma64<PASS, D>), dim3(4 * blocks), dim3(64), 0, st, wis, xs, gidx, \
                      data, XB, tout, Y, yr, yc, nrhs, part, perm, dof, rowmap)
+#define BW_MFMA64R(D)                                                                         \
+  hipLaunchKernelGGL((k_slab_mfma64<PASS, D, true>), dim3(4 * blocks), dim3(64), 0, st, wis, xs, \
+                     gidx, data, XB, tout, Y, yr, yc, nrhs, part, perm, dof, rowmap)
   switch (schedule) {
     case 4: BW_MFMA64(3); break;
     case 5: BW_MFMA64(4); break;
     case 6: BW_MFMA64(6); break;
     case 7: BW_MFMA64(8); break;
+    case 8: BW_MFMA64R(3); break;
+    case 9: BW_MFMA64R(4); break;
+    case 10: BW_MFMA64R(6); break;
+    case 11: BW_MFMA64R(8); break;
+    case 12: BW_MFMA64R(3); break;  // the FP64 kernel has 16-row tiles already
+    case 13: BW_MFMA64R(4); break;
     default: BW_MFMA64(0);
   }
 #undef BW_MFMA64
+#undef BW_MFMA64R
 }
 
 // ---------------------------------------------------------------------------
@@ -1181,6 +1498,16 @@ int undo_allocs(StreamCtx* ctx, size_t mark, int rc) {
   return rc;
 }
 
+// Zeroes fresh scratch and waits for it, so that no kernel of any stream can
+// see it before the zeros land.  (hipMemset of device memory returns before
+// it completes, on the null stream, which does not order non-blocking
+// streams such as torch's.)
+hipError_t zero_now(void* p, size_t bytes) {
+  hipError_t e = hipMemsetAsync(p, 0, bytes, nullptr);
+  if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+  return e;
+}
+
 // Allocates what `ctx` lacks for operator [adj][fp32]: the single-vector
 // scratch, and with `batched` the kRhs-wide one.  xt / xb are zeroed here,
 // once (synchronously, before any kernel of the context can use them): the
@@ -1199,9 +1526,9 @@ int ensure_op_scratch(bw_hip_handle* hd, StreamCtx* ctx, int adj, int fp32, bool
     for (int p = 0; p < 2; ++p)
       if ((rc = dev_alloc(&part[p], op.part_elems[p] * es, &ctx->allocs, "scratch")))
         return undo_allocs(ctx, mark, rc);
-    const hipError_t e = hipMemset(xt, 0, op.xt_size * es);
+    const hipError_t e = zero_now(xt, op.xt_size * es);
     if (e != hipSuccess)
-      return undo_allocs(ctx, mark, fail(BW_EHIP, std::string("scratch: hipMemset: ") + hipGetErrorString(e)));
+      return undo_allocs(ctx, mark, fail(BW_EHIP, std::string("scratch: zero fill: ") + hipGetErrorString(e)));
     s.xt = xt;
     s.part[0] = part[0];
     s.part[1] = part[1];
@@ -1216,9 +1543,9 @@ int ensure_op_scratch(bw_hip_handle* hd, StreamCtx* ctx, int adj, int fp32, bool
       if ((rc = dev_alloc(&partb[p], static_cast<size_t>(op.part_elems[p]) * kRhs * es, &ctx->allocs,
                           "scratch")))
         return undo_allocs(ctx, mark_b, rc);
-    const hipError_t e = hipMemset(xb, 0, static_cast<size_t>(op.xt_size) * kRhs * es);
+    const hipError_t e = zero_now(xb, static_cast<size_t>(op.xt_size) * kRhs * es);
     if (e != hipSuccess)
-      return undo_allocs(ctx, mark_b, fail(BW_EHIP, std::string("scratch: hipMemset: ") + hipGetErrorString(e)));
+      return undo_allocs(ctx, mark_b, fail(BW_EHIP, std::string("scratch: zero fill: ") + hipGetErrorString(e)));
     s.xb = xb;
     s.partb[0] = partb[0];
     s.partb[1] = partb[1];
@@ -1908,6 +2235,9 @@ int check_batched(bw_hip_handle* hd, bool adjoint, int nrhs, const double* X, in
                                     : "matvec_batched: precision not built");
   if (leading_dims && (xc < op.n_in || yc < op.n_out))
     return fail(BW_ESHAPE, "matvec_batched: leading dimension too small");
+  // the MFMA kernels index the kRhs-wide input rows with 32-bit offsets
+  if (op.xt_size * kRhs > static_cast<int64_t>(UINT32_MAX))
+    return fail(BW_ESHAPE, "matvec_batched: input plus rank entries exceed 2^32 / 32");
   // distinct outputs must not share an address: columns apart by at least a
   // column's extent, or rows apart by at least a row's
   if (!(yc >= op.n_out * yr || yr >= static_cast<int64_t>(nrhs) * yc))
@@ -2199,7 +2529,7 @@ int bw_hip_set_unroll_pass(bw_hip_handle* hd, int precision_mask, int pass, int 
 int bw_hip_set_mfma_schedule(bw_hip_handle* hd, int precision_mask, int schedule) {
   if (!hd) return fail(BW_EINVAL, "set_mfma_schedule: null handle");
   std::lock_guard<std::mutex> lock(hd->mu);
-  if (schedule < 0 || schedule > 7) return fail(BW_EINVAL, "set_mfma_schedule: schedule must be 0-7");
+  if (schedule < 0 || schedule > 13) return fail(BW_EINVAL, "set_mfma_schedule: schedule must be 0-13");
   if (precision_mask <= 0 || (precision_mask & ~(BW_PREC_FP64 | BW_PREC_FP32)))
     return fail(BW_EINVAL, "set_mfma_schedule: bad precision mask");
   for (int adj = 0; adj < 2; ++adj) {

@@ -108,7 +108,7 @@ int bw_hip_matvec(bw_hip_handle* hd, const double* x, double* y, int ordering,
  * (one stream, or events).  Each call clears what must start at zero.
  * x and y are device pointers; selections return BW_ENOTSUP; the per-kernel
  * timers do not record these calls. */
-#define BW_NOALLOC_SCRATCH_BYTES (64LL << 20)
+#define BW_NOALLOC_SCRATCH_BYTES (128LL << 20)
 int bw_hip_matvec_noalloc(bw_hip_handle* hd, const double* x, double* y, int ordering,
                           int precision, void* stream);
 
@@ -253,10 +253,16 @@ int bw_hip_set_unroll_pass(bw_hip_handle* hd, int precision_mask, int pass, int 
  * loads are issued before the current step's MFMAs (0: 4 pairs; 1: 2 pairs,
  * pipelined; 2: 4 pairs, pipelined; 3: 8 pairs; 4-7: a ring of D step
  * buffers with D - 1 steps of loads in flight: 4: 2 pairs, D = 3; 5: 2 pairs,
- * D = 4; 6: 1 pair, D = 6; 7: 1 pair, D = 8).  FP64: 0-3 load two k-steps
- * before their MFMAs; 4-7 the ring with D = 3, 4, 6, 8.  Every schedule gives
- * the same bits.  Default: 4 (FP32), 5 (FP64); bigwham_dev_amd.autotune
- * times them all on the caller's matrix. */
+ * D = 4; 6: 1 pair, D = 6; 7: 1 pair, D = 8; 8-11: the same four shapes as a
+ * ring consumed in load order, every buffer refilled right after its MFMAs,
+ * D - 1 steps in flight at every step; 12, 13: 16-row tiles on
+ * v_mfma_f32_16x16x4f32, ceil(R / 16) tiles per slab of R rows, load-order
+ * ring with D = 3, 4).  FP64: 0-3 load two k-steps before their MFMAs; 4-7
+ * the ring with D = 3, 4, 6, 8; 8-11 the load-order ring with D = 3, 4, 6, 8
+ * (12, 13 = 8, 9: the FP64 kernel has 16-row tiles throughout).  Every
+ * schedule gives the same values to FP32 / FP64 rounding of one summation
+ * order per 16- or 32-row tile.  Default: 4 (FP32), 5 (FP64);
+ * bigwham_dev_amd.autotune times them all on the caller's matrix. */
 int bw_hip_set_mfma_schedule(bw_hip_handle* hd, int precision_mask, int schedule);
 
 /* Per-kernel timing.  With timing enabled every matvec records HIP events

@@ -26,7 +26,7 @@ import os
 from dataclasses import dataclass, field
 
 UNROLLS = (4, 8, 16)
-MFMA_SCHEDULES = (0, 1, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13)   # 2 spills registers on gfx950
+MFMA_SCHEDULES = (0, 1, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15)   # 2 spills registers on gfx950
 MFMA64_SCHEDULES = (0, 4, 5, 6, 7, 8, 9, 10, 11)       # 1-3 are 0 for FP64
 WI_BYTES = (64 << 10, 96 << 10, 192 << 10, 384 << 10)
 

@@ -637,7 +637,92 @@ __device__ __forceinline__ void mfma_stream_ring(const flt4* __restrict__ base,
   __builtin_amdgcn_sched_barrier(0);
 }
 
-template <int PASS, int KU, bool PIPE, int D = 0, bool RING = false>
+// The load-order ring with the B operands staged through LDS: a column group's
+// four input rows (4 x kRhs floats, 512 contiguous bytes) arrive as one
+// 16-byte load per lane of its half-wave instead of four 4-byte loads per
+// lane, are written to the wave's LDS slot and read back in the MFMA layout
+// (row s, column lane % 32).  Steps go: MFMAs of step s (B in registers, read
+// from LDS the stage before), then step s + 1's B written to LDS and read
+// back, then the refill of step s + D.  One wave per workgroup, so the slot
+// is the wave's own; its LDS operations run in order, so a read sees the
+// write before it.
+template <int KU, int D, bool TWO>
+__device__ __forceinline__ void mfma_stream_lds(const flt4* __restrict__ base,
+                                                const int32_t* __restrict__ gi,
+                                                const float* __restrict__ XB, int ng, int R,
+                                                int half, int r0, float* slot, floatx16& acc0,
+                                                floatx16& acc1) {
+  constexpr int G = 2 * KU;  // groups per step
+  const int rA0 = min(r0, R - 1), rA1 = min(r0 + 32, R - 1);
+  const int nsteps = (ng + G - 1) / G;
+  flt4 a0[D][KU], a1[D][KU];
+  flt4 bq[D][KU];  // this lane's 16 bytes of the four input rows of group 2u + half
+  int zz[D][KU];    // input offset of group 2u + half for the step buffer k gets next
+  float bn[KU][4];  // B of the next step to multiply, read back from LDS
+  auto gz = [&](int step, int u) { return gi[min(step * G + 2 * u + half, ng - 1)]; };
+  auto load = [&](int step, const int (&z)[KU], flt4 (&x0)[KU], flt4 (&x1)[KU], flt4 (&bb)[KU]) {
+#pragma unroll
+    for (int u = 0; u < KU; ++u) {
+      const int ga = min(step * G + 2 * u + half, ng - 1);
+      x0[u] = __builtin_nontemporal_load(base + ga * R + rA0);
+      if (TWO) x1[u] = __builtin_nontemporal_load(base + ga * R + rA1);
+      // rows z .. z + 3 of XB are 512 contiguous, 128-byte aligned bytes
+      const uint32_t zo = static_cast<uint32_t>(z[u]) * kRhs + 4 * r0;
+      bb[u] = *reinterpret_cast<const flt4*>(XB + zo);
+    }
+  };
+  auto stage_b = [&](const flt4 (&bb)[KU]) {
+#pragma unroll
+    for (int u = 0; u < KU; ++u)
+      *reinterpret_cast<flt4*>(slot + (2 * u + half) * 4 * kRhs + 4 * r0) = bb[u];
+#pragma unroll
+    for (int u = 0; u < KU; ++u)
+#pragma unroll
+      for (int s = 0; s < 4; ++s) bn[u][s] = slot[(2 * u + half) * 4 * kRhs + s * kRhs + r0];
+  };
+  auto compute = [&](int step, const flt4 (&x0)[KU], const flt4 (&x1)[KU]) {
+#pragma unroll
+    for (int u = 0; u < KU; ++u) {
+      const bool ok = step * G + 2 * u + half < ng;
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const float bv = ok ? bn[u][s] : 0.0f;
+        acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(x0[u][s], bv, acc0, 0, 0, 0);
+        if (TWO) acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(x1[u][s], bv, acc1, 0, 0, 0);
+      }
+    }
+  };
+#pragma unroll
+  for (int k = 0; k < D; ++k)
+#pragma unroll
+    for (int u = 0; u < KU; ++u) zz[k][u] = gz(k, u);
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    __builtin_amdgcn_sched_barrier(0);
+    load(k, zz[k], a0[k], a1[k], bq[k]);
+#pragma unroll
+    for (int u = 0; u < KU; ++u) zz[k][u] = gz(k + D, u);
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  stage_b(bq[0]);
+  for (int s0 = 0; s0 < nsteps; s0 += D) {
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      const int s = s0 + j;
+      __builtin_amdgcn_sched_barrier(0);
+      compute(s, a0[j], a1[j]);  // past the last step: B = 0, adds nothing
+      __builtin_amdgcn_sched_barrier(0);
+      stage_b(bq[(j + 1) % D]);
+      __builtin_amdgcn_sched_barrier(0);
+      load(s + D, zz[j], a0[j], a1[j], bq[j]);
+#pragma unroll
+      for (int u = 0; u < KU; ++u) zz[j][u] = gz(s + 2 * D, u);
+    }
+  }
+  __builtin_amdgcn_sched_barrier(0);
+}
+
+template <int PASS, int KU, bool PIPE, int D = 0, bool RING = false, bool LDSB = false>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_slab_mfma(
     const bwl::WorkItem* __restrict__ wis, XcdSplit xs, const int32_t* __restrict__ gidx,
     const float* __restrict__ data, const float* __restrict__ XB,
@@ -649,16 +734,22 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
   const int w = xcd_work_item<1>(xs);
   if (w < 0) return;
   const bwl::WorkItem wi = wis[w];
-  const int R = wi.rows;
-  const int ng = wi.ngroups;
+  const int R = __builtin_amdgcn_readfirstlane(wi.rows);
+  const int ng = __builtin_amdgcn_readfirstlane(wi.ngroups);
   const int half = lane >> 5;  // which group of the pair
   const int r0 = lane & 31;
   const flt4* __restrict__ base = reinterpret_cast<const flt4*>(data + wi.data_off);
-  const int32_t* __restrict__ gi = gidx + wi.gbeg;
+  const int32_t* __restrict__ gi = gidx + __builtin_amdgcn_readfirstlane(wi.gbeg);
   floatx16 acc0 = {0}, acc1 = {0};
   const bool two = R > 32;
   if (ng > 0) {
-    if constexpr (RING) {
+    if constexpr (LDSB) {
+      __shared__ __align__(16) float slot[2 * KU * 4 * kRhs];
+      if (two)
+        mfma_stream_lds<KU, D, true>(base, gi, XB, ng, R, half, r0, slot, acc0, acc1);
+      else
+        mfma_stream_lds<KU, D, false>(base, gi, XB, ng, R, half, r0, slot, acc0, acc1);
+    } else if constexpr (RING) {
       if (two)
         mfma_stream_ring<KU, D, true>(base, gi, XB, ng, R, half, r0, acc0, acc1);
       else
@@ -893,6 +984,12 @@ void launch_mfma(int schedule, int blocks, hipStream_t st, const bwl::WorkItem* 
     case 10: BW_MFMA32_RING(1, 6); break;
     case 11: BW_MFMA32_RING(1, 8); break;
 #undef BW_MFMA32_RING
+#define BW_MFMA32_LDSB(KU, D)                                                                    \
+  hipLaunchKernelGGL((k_slab_mfma<PASS, KU, true, D, true, true>), dim3(4 * blocks), dim3(64), 0, \
+                     st, wis, xs, gidx, data, XB, tout, Y, yr, yc, nrhs, part, perm, dof, rowmap)
+    case 14: BW_MFMA32_LDSB(2, 3); break;
+    case 15: BW_MFMA32_LDSB(2, 4); break;
+#undef BW_MFMA32_LDSB
     case 12:
       hipLaunchKernelGGL((k_slab_mfma16<PASS, 3>), dim3(4 * blocks), dim3(64), 0, st, wis, xs, gidx,
                          data, XB, tout, Y, yr, yc, nrhs, part, perm, dof, rowmap);
@@ -1150,6 +1247,8 @@ void launch_mfma64(int schedule, int blocks, hipStream_t st, const bwl::WorkItem
     case 11: BW_MFMA64R(8); break;
     case 12: BW_MFMA64R(3); break;  // the FP64 kernel has 16-row tiles already
     case 13: BW_MFMA64R(4); break;
+    case 14: BW_MFMA64R(3); break;  // FP32-only B staging: the FP64 ring
+    case 15: BW_MFMA64R(4); break;
     default: BW_MFMA64(0);
   }
 #undef BW_MFMA64
@@ -1185,7 +1284,7 @@ struct DevOp {
   int64_t stream_bytes = 0;
   int64_t part_elems[2] = {0, 0};
   int unroll[2] = {16, 16};   // loads in flight per lane in k_slab (4, 8 or 16), per pass
-  int mfma_schedule = 0;      // k_slab_mfma / k_slab_mfma64 variant (0-7)
+  int mfma_schedule = 0;      // k_slab_mfma / k_slab_mfma64 variant (0-15)
 };
 
 // What one matvec of an operator writes: xt (x gathered in the factor
@@ -1470,9 +1569,10 @@ int build_op(bw_hip_handle* hd, const bw_flat_hmat* h, bool adjoint, hipStream_t
   if ((rc = upload_pass<T>(L, L.pass1, &op.p1, st, &hd->allocs))) return rc;
   if ((rc = upload_pass<T>(L, L.pass2, &op.p2, st, &hd->allocs))) return rc;
   op.elem = sizeof(T);
-  // default batched schedule: the one the autotuner picked on MI355X for the
-  // 150k-DoF 3DR0 matrix (FP32: two pairs, ring of 3; FP64: ring of 4)
-  op.mfma_schedule = sizeof(T) == 8 ? 5 : 4;
+  // default batched schedule: the fastest on MI355X for the 150k-DoF 3DR0
+  // matrix (FP32: two pairs, load-order ring of 3, B staged through LDS;
+  // FP64: ring of 4)
+  op.mfma_schedule = sizeof(T) == 8 ? 5 : 14;
   op.xt_size = L.xt_size;
   op.xt_t = L.xt_t;
   op.t_total = L.view.t_total;
@@ -2529,7 +2629,7 @@ int bw_hip_set_unroll_pass(bw_hip_handle* hd, int precision_mask, int pass, int 
 int bw_hip_set_mfma_schedule(bw_hip_handle* hd, int precision_mask, int schedule) {
   if (!hd) return fail(BW_EINVAL, "set_mfma_schedule: null handle");
   std::lock_guard<std::mutex> lock(hd->mu);
-  if (schedule < 0 || schedule > 13) return fail(BW_EINVAL, "set_mfma_schedule: schedule must be 0-13");
+  if (schedule < 0 || schedule > 15) return fail(BW_EINVAL, "set_mfma_schedule: schedule must be 0-15");
   if (precision_mask <= 0 || (precision_mask & ~(BW_PREC_FP64 | BW_PREC_FP32)))
     return fail(BW_EINVAL, "set_mfma_schedule: bad precision mask");
   for (int adj = 0; adj < 2; ++adj) {

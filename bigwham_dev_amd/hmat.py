@@ -399,7 +399,7 @@ class HipHmat:
                                                    int(unroll)))
 
     def set_mfma_schedule(self, schedule: int, precision: str = "fp32") -> None:
-        """Schedule (0-13) of the batched MFMA kernel of one precision."""
+        """Schedule (0-15) of the batched MFMA kernel of one precision."""
         N.check_hip(N.hip().bw_hip_set_mfma_schedule(self._h, self._prec(precision),
                                                      int(schedule)))
 

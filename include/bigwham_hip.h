@@ -257,11 +257,13 @@ int bw_hip_set_unroll_pass(bw_hip_handle* hd, int precision_mask, int pass, int 
  * ring consumed in load order, every buffer refilled right after its MFMAs,
  * D - 1 steps in flight at every step; 12, 13: 16-row tiles on
  * v_mfma_f32_16x16x4f32, ceil(R / 16) tiles per slab of R rows, load-order
- * ring with D = 3, 4).  FP64: 0-3 load two k-steps before their MFMAs; 4-7
+ * ring with D = 3, 4; 14, 15: schedules 8, 9 with each column group's four
+ * input rows loaded as one 16-byte load per lane and handed to the MFMA
+ * layout through LDS).  FP64: 0-3 load two k-steps before their MFMAs; 4-7
  * the ring with D = 3, 4, 6, 8; 8-11 the load-order ring with D = 3, 4, 6, 8
- * (12, 13 = 8, 9: the FP64 kernel has 16-row tiles throughout).  Every
+ * (12, 14 = 8 and 13, 15 = 9: the FP64 kernel has 16-row tiles throughout).  Every
  * schedule gives the same values to FP32 / FP64 rounding of one summation
- * order per 16- or 32-row tile.  Default: 4 (FP32), 5 (FP64);
+ * order per 16- or 32-row tile.  Default: 14 (FP32), 5 (FP64);
  * bigwham_dev_amd.autotune times them all on the caller's matrix. */
 int bw_hip_set_mfma_schedule(bw_hip_handle* hd, int precision_mask, int schedule);
 

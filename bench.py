@@ -405,6 +405,45 @@ def _timed(fn, steps, warmup):
     return (time.perf_counter() - t0) / steps
 
 
+def dense_gemm_reference(E, k, reps=10):
+    """The vendor BLAS (torch.mm on ROCm, TF32 off) on a dense GEMM with the
+    same bytes and flops as the 32-RHS call: A (M x K, M*K = E) times
+    B (K x k), for two shapes of A in both storage orders.  A reference point
+    for what a matrix-core GEMM of this arithmetic intensity reaches on the
+    chip, not a bound."""
+    import torch
+    torch.backends.cuda.matmul.allow_tf32 = False
+    out = {}
+    for dt, name in ((torch.float32, "fp32"), (torch.float64, "fp64")):
+        runs = {}
+        for M in (32768, 150528):
+            K = E // M
+            for colmajor in (False, True):
+                g = torch.Generator(device="cuda:0")
+                g.manual_seed(1)
+                A = torch.rand((K, M) if colmajor else (M, K), device="cuda:0", dtype=dt,
+                               generator=g) - 0.5
+                B = torch.rand((K, k), device="cuda:0", dtype=dt, generator=g) - 0.5
+                Am = A.t() if colmajor else A
+                for _ in range(3):
+                    torch.mm(Am, B)
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record()
+                for _ in range(reps):
+                    torch.mm(Am, B)
+                e1.record()
+                torch.cuda.synchronize()
+                runs[f"M{M}_{'colmajor' if colmajor else 'rowmajor'}"] = round(
+                    e0.elapsed_time(e1) / reps, 4)
+                del A, B, Am
+                torch.cuda.empty_cache()
+        best = min(runs.values())
+        out[name] = {"ms": runs, "best_ms": best,
+                     "best_tflops": round(2.0 * k * E / (best * 1e-3) / 1e12, 1)}
+    return out
+
+
 def bench_batching(args, cfg):
     """Many right-hand sides (FP32 MFMA thin GEMM) and many meshes in one
     operator.  value = right-hand sides per second of the 32-RHS FP32 call."""
@@ -515,6 +554,10 @@ def bench_batching(args, cfg):
                                           torch.linalg.matrix_norm(Ycat))}
         gb.close()
     log(f"16 meshes x 32 RHS: {({p: res[p]['batched_32rhs'] for p in res})}")
+    dense = dense_gemm_reference(E, k)
+    dense["fp32"]["dense_best_over_h_call"] = round(dense["fp32"]["best_ms"] / (t_b32 * 1e3), 3)
+    dense["fp64"]["dense_best_over_h_call"] = round(dense["fp64"]["best_ms"] / (t_b64 * 1e3), 3)
+    log(f"dense GEMM of the same size through the vendor BLAS: {dense}")
 
     def roof(t, es, peak_tf, nbytes):
         # attainable roof: the slower of the MFMA peak and HBM at 8 TB/s
@@ -546,6 +589,7 @@ def bench_batching(args, cfg):
                          "speedup_vs_32_fp64_calls": round(k * t_s64 / t_b64, 2)},
         "parity": {"batched_fp32_rel_err_vs_fp64": err, "batched_fp64_rel_err_vs_fp64": err64},
         "multi_mesh": {"sides": sides, "dof": sizes, "nb_entries": e_multi, **res},
+        "dense_gemm_reference": dense,
         "setup_s": {"host_build": round(t_build, 2), "multi_build": round(t_mbuild, 2)},
         "tuning": tuning,
     }

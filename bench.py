@@ -126,11 +126,41 @@ def cpu_baseline(host, x, budget_s=15.0):
     med = float(np.median(ts))
     agree = float(np.linalg.norm(y_native - y) / np.linalg.norm(y))
     flags = " ".join(build_oracle.NATIVE_FLAGS)
-    return y, dict(value=1.0 / med, unit="H-matvec/s", cores=threads, kind="port",
+
+    def host_matvec(v):
+        return O.matvec_original(flat, v, threads, use=native)
+    host_matvec.flat = flat  # keeps the flat view's buffers alive with the closure
+    return y, host_matvec, dict(value=1.0 / med, unit="H-matvec/s", cores=threads, kind="port",
                    sample=f"{reps} timed calls (+1 warm-up) of the full matvec, median "
                           f"{med * 1e3:.2f} ms, OpenMP restatement of Hmat::matvecOriginal "
                           f"built with gcc {flags} on this host ({threads} threads); "
                           f"rel. diff to the reference build {agree:.1e}")
+
+
+def host_gmres(host_matvec, diag, b, tol, restart):
+    """The host path's solve, as a user of the reference runs it: scipy's GMRES
+    on a LinearOperator whose matvec is the CPU baseline's OpenMP matvec, with
+    the Jacobi preconditioner 1/diag(H) (bigwham4py.py:373-380), restart 30.
+    Returns the wall time, the matvec count and the true FP64 residual."""
+    from scipy.sparse import diags
+    from scipy.sparse.linalg import LinearOperator, gmres as sp_gmres
+    n = b.size
+    count = [0]
+
+    def mv(v):
+        count[0] += 1
+        return host_matvec(np.ascontiguousarray(v, dtype=np.float64).ravel())
+    A = LinearOperator((n, n), matvec=mv, dtype=np.float64)
+    M = diags(1.0 / diag)
+    t0 = time.perf_counter()
+    x, flag = sp_gmres(A, b, rtol=tol, atol=0.0, restart=restart, maxiter=100, M=M)
+    dt = time.perf_counter() - t0
+    nmv = count[0]
+    rel = float(np.linalg.norm(b - host_matvec(x)) / np.linalg.norm(b))
+    log(f"host GMRES: {dt:.1f} s, {nmv} matvecs, scipy flag {flag}, rel. residual {rel:.2e}")
+    return {"seconds": round(dt, 2), "matvecs": nmv, "scipy_info": int(flag),
+            "rel_residual_fp64": rel, "solver": "scipy.sparse.linalg.gmres, restart 30, "
+            "Jacobi 1/diag, OpenMP restatement of Hmat::matvecOriginal"}
 
 
 def dense_check(host, dev, x, max_dense=12000, blocks=8, m=8):
@@ -187,6 +217,9 @@ def main():
                     help="only run a plain 1 GiB device copy (calibrates HBM counters)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default unroll depth instead of timing 4/8/16 first")
+    ap.add_argument("--host-gmres", action="store_true",
+                    help="with --gmres: also run the host path's GMRES to 1e-8 (scipy GMRES on "
+                         "the CPU baseline's OpenMP matvec, Jacobi; about 200 host matvecs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: bigwham_dev_amd runs on one GPU only; multi-GPU sharding is a design "
@@ -338,7 +371,7 @@ def main():
     }
 
     if not args.no_cpu:
-        y_ref, cb = cpu_baseline(host, x_np)
+        y_ref, host_matvec, cb = cpu_baseline(host, x_np)
         out["cpu_baseline"] = cb
         out["parity"] = {
             "fp64_rel_err_vs_oracle": float(np.linalg.norm(y64 - y_ref) / np.linalg.norm(y_ref)),
@@ -383,6 +416,11 @@ def main():
                 "seconds_lower_bound": round(t_host, 2),
                 "basis": f"{info64.matvecs_fp64} FP64 matvecs x host matvec time",
                 "speedup_mixed_vs_host": round(t_host / infom.seconds, 1)}
+            if args.host_gmres:
+                out["gmres"]["host_reference"] = host_gmres(
+                    host_matvec, dev.diagonal(original=True), b.cpu().numpy(), 1e-8, 30)
+                out["gmres"]["host_reference"]["speedup_mixed_vs_host"] = round(
+                    out["gmres"]["host_reference"]["seconds"] / infom.seconds, 1)
     print(json.dumps(out), flush=True)
 
 

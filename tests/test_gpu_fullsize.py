@@ -3,8 +3,11 @@
 penny crack of ~200 000 triangles (599 739 DoF), FP64 against the oracle
 (restatement of Hmat::matvecOriginal, src/hmat/hmatrix/hmat.cc:572-715) at
 1e-12, FP32 at 1e-5, plus size-independent properties: the adjoint identity,
-linearity and bitwise determinism; and the 3DR0-H 224x224 crack (150 528 DoF)
-with 32 right-hand sides through the batched MFMA path."""
+linearity and bitwise determinism; the gradient of ||H w - t||^2 through
+torch.autograd against the oracle's 2 H^T (H w - t) and central differences;
+the penny's FP64 and mixed-precision GMRES to 1e-8 with the residual taken
+by the oracle; and the 3DR0-H 224x224 crack (150 528 DoF) with 32
+right-hand sides through the batched MFMA path."""
 import numpy as np
 import pytest
 
@@ -44,17 +47,46 @@ def test_full_size(gpu, kernel, n):
     y2 = dev.matvec(2.0 * xd - 3.0 * w)
     assert float(torch.linalg.vector_norm(y2 - (2.0 * y - 3.0 * dev.matvec(w))) /
                  torch.linalg.vector_norm(y2)) <= 1e-13
+    # gradient of ||H w - t||^2 through the adjoint kernel: the oracle's
+    # 2 H^T (H w - t) at 1e-12, and central differences in 4 directions
+    # (the loss is quadratic, so they are exact up to rounding)
+    from bigwham_dev_amd.autograd import hmatvec
+    t_np = ref + 0.01 * np.abs(ref).mean() * rng.standard_normal(h.size)
+    w_np = 0.5 * x
+    td = torch.from_numpy(t_np).to(gpu)
+    wv = torch.from_numpy(w_np).to(gpu).requires_grad_(True)
+
+    def loss_fn(ww):
+        r = hmatvec(dev, ww) - td
+        return (r * r).sum()
+
+    loss = loss_fn(wv)
+    loss.backward()
+    g_ref = 2.0 * O.rmatvec_original(f, O.matvec_original(f, w_np) - t_np)
+    assert rel(wv.grad.cpu().numpy(), g_ref) <= 1e-12
+    with torch.no_grad():
+        for _ in range(4):
+            v = torch.from_numpy(rng.standard_normal(h.size)).to(gpu)
+            eps = 1e-3 * torch.linalg.vector_norm(wv) / torch.linalg.vector_norm(v)
+            fd = (loss_fn(wv + eps * v) - loss_fn(wv - eps * v)) / (2 * eps)
+            an = (wv.grad * v).sum()
+            assert abs(fd.item() - an.item()) <= 1e-7 * abs(an.item()) + 1e-9 * loss.item()
 
 
 def test_full_size_penny(gpu):
-    """The 600k-DoF 3DT0-H penny matrix (FP64 + FP32 copies: 80 GB of HBM)."""
+    """The 600k-DoF 3DT0-H penny matrix (FP64 copy and the FP32 copy with its
+    low-rank blocks recompressed to 1e-7, as benchmarked: 76 GB of HBM); then
+    the benchmarked solves: GMRES to 1e-8 with the Jacobi preconditioner,
+    FP64 and mixed precision, each checked with the oracle's FP64 matvec."""
     import torch
+    from bigwham_dev_amd import solvers
     print("full size 3DT0-H penny, 105 000 points", flush=True)
     coor, conn = penny_delaunay(105000, seed=0)
     h = HostHmat.from_mesh("3DT0-H", coor, conn, [1.0, 0.25])
     h.build(64, 3.0, 1e-6)
     print(f"built N={h.size} E={h.nb_entries()}", flush=True)
-    dev = HipHmat(h, precisions=("fp64", "fp32"))
+    dev = HipHmat(h, precisions=("fp64", "fp32"), fp32_rank_tol=1e-7)
+    print("uploaded (FP32 copy recompressed)", flush=True)
     rng = np.random.default_rng(1234)
     x = rng.uniform(-1, 1, h.size)
     ref = O.matvec_original(h.flat(), x)
@@ -67,6 +99,17 @@ def test_full_size_penny(gpu):
     y2 = dev.matvec(2.0 * xd - 3.0 * w)
     assert float(torch.linalg.vector_norm(y2 - (2.0 * y - 3.0 * dev.matvec(w))) /
                  torch.linalg.vector_norm(y2)) <= 1e-13
+    b = torch.zeros(h.size, dtype=torch.float64, device=gpu)
+    b[h.dof - 1::h.dof] = 1.0  # unit normal pressure (normal component last)
+    bn = b.cpu().numpy()
+    x64, i64 = solvers.solve_fp64(dev, b, tol=1e-8)
+    xm, im = solvers.solve_mixed(dev, b, tol=1e-8)
+    print(f"gmres fp64 {i64.iterations} its, mixed {im.matvecs_fp32} fp32 + "
+          f"{im.matvecs_fp64} fp64 matvecs", flush=True)
+    for sol, info in ((x64, i64), (xm, im)):
+        r = rel(O.matvec_original(h.flat(), sol.cpu().numpy()), bn)
+        assert info.converged and r <= 1e-8, r
+    assert float(torch.linalg.vector_norm(xm - x64) / torch.linalg.vector_norm(x64)) <= 1e-6
 
 
 def test_full_size_batched(gpu):

@@ -379,6 +379,41 @@ __global__ __launch_bounds__(256) void k_gather_b(const double* __restrict__ X, 
   XB[i * kRhs + c] = c < nrhs ? static_cast<T>(X[src * xr + c * xc]) : T(0);
 }
 
+// The same gather for a row-major X (unit column stride, even row stride,
+// 16-byte aligned): eight threads per row, four columns each, two 16-byte
+// loads and one 16-byte (FP32) or two (FP64) stores per thread.
+template <typename T>
+__global__ __launch_bounds__(256) void k_gather_b4(const double* __restrict__ X, int64_t xr,
+                                                   int nrhs, T* __restrict__ XB, int64_t n,
+                                                   const int64_t* __restrict__ perm, int dof) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * (blockDim.x / 8) + threadIdx.x / 8;
+  const int c0 = 4 * (threadIdx.x % 8);
+  if (i >= n) return;
+  int64_t src = i;
+  if (perm) {
+    const int64_t p = i / dof;
+    src = dof * perm[p] + (i - p * dof);
+  }
+  const double* __restrict__ row = X + src * xr + c0;
+  double v[4];
+  if (c0 + 4 <= nrhs) {
+    const dbl2 a = *reinterpret_cast<const dbl2*>(row);
+    const dbl2 b = *reinterpret_cast<const dbl2*>(row + 2);
+    v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
+  } else {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = c0 + k < nrhs ? row[k] : 0.0;
+  }
+  T* dst = XB + i * kRhs + c0;
+  if constexpr (sizeof(T) == 4) {
+    *reinterpret_cast<flt4*>(dst) = flt4{static_cast<float>(v[0]), static_cast<float>(v[1]),
+                                         static_cast<float>(v[2]), static_cast<float>(v[3])};
+  } else {
+    *reinterpret_cast<dbl2*>(dst) = dbl2{v[0], v[1]};
+    *reinterpret_cast<dbl2*>(dst + 2) = dbl2{v[2], v[3]};
+  }
+}
+
 // KU group pairs per step; PIPE: the loads of step i+1 are issued before the
 // MFMAs of step i.  Addresses are clamped into the work item, so every lane
 // loads valid data; a group past the end gets B = 0 (its A rows then add
@@ -2359,14 +2394,22 @@ int launch_batched(bw_hip_handle* hd, DevOp& op, const OpScratch& s, int nrhs, c
     double* Yk = Y + static_cast<int64_t>(k0) * yc;
     const int64_t rows_per_block = 256 / kRhs;
     const int gblocks = static_cast<int>((op.n_in + rows_per_block - 1) / rows_per_block);
+    // row-major input (unit column stride, even row stride, 16-byte aligned
+    // rows): the vector gather, 32 rows per block
+    const bool rowmajor = xc == 1 && xr % 2 == 0 && reinterpret_cast<uintptr_t>(Xk) % 16 == 0;
+    const int gblocks4 = static_cast<int>((op.n_in + 31) / 32);
     const int64_t* pin = orig ? op.perm_in : nullptr;
     const int64_t* pout = orig ? op.perm_out : nullptr;
     if (fp64) {
       double* xb = static_cast<double*>(s.xb);
       double* pb0 = static_cast<double*>(s.partb[0]);
       double* pb1 = static_cast<double*>(s.partb[1]);
-      hipLaunchKernelGGL(k_gather_b<double>, dim3(gblocks), dim3(256), 0, st, Xk, xr, xc, nk, xb,
-                         op.n_in, pin, dof);
+      if (rowmajor)
+        hipLaunchKernelGGL(k_gather_b4<double>, dim3(gblocks4), dim3(256), 0, st, Xk, xr, nk, xb,
+                           op.n_in, pin, dof);
+      else
+        hipLaunchKernelGGL(k_gather_b<double>, dim3(gblocks), dim3(256), 0, st, Xk, xr, xc, nk, xb,
+                           op.n_in, pin, dof);
       if (op.p1.nwi > 0)
         launch_mfma64<1>(op.mfma_schedule, op.p1.blocks, st, op.p1.wis, op.p1.xs, op.p1.gidx,
                          static_cast<const double*>(op.p1.data), xb, xb, nullptr, 0, 0, kRhs,
@@ -2385,8 +2428,12 @@ int launch_batched(bw_hip_handle* hd, DevOp& op, const OpScratch& s, int nrhs, c
       float* xb = static_cast<float*>(s.xb);
       float* pb0 = static_cast<float*>(s.partb[0]);
       float* pb1 = static_cast<float*>(s.partb[1]);
-      hipLaunchKernelGGL(k_gather_b<float>, dim3(gblocks), dim3(256), 0, st, Xk, xr, xc, nk, xb,
-                         op.n_in, pin, dof);
+      if (rowmajor)
+        hipLaunchKernelGGL(k_gather_b4<float>, dim3(gblocks4), dim3(256), 0, st, Xk, xr, nk, xb,
+                           op.n_in, pin, dof);
+      else
+        hipLaunchKernelGGL(k_gather_b<float>, dim3(gblocks), dim3(256), 0, st, Xk, xr, xc, nk, xb,
+                           op.n_in, pin, dof);
       if (op.p1.nwi > 0)
         launch_mfma<1>(op.mfma_schedule, op.p1.blocks, st, op.p1.wis, op.p1.xs, op.p1.gidx,
                        static_cast<const float*>(op.p1.data), xb, xb, nullptr, 0, 0, kRhs, pb0,

@@ -444,16 +444,17 @@ def _timed(fn, steps, warmup):
     return (time.perf_counter() - t0) / steps
 
 
-def dense_gemm_reference(E, k, reps=10):
+def dense_gemm_reference(E64, E32, k, reps=10):
     """The vendor BLAS (torch.mm on ROCm, TF32 off) on a dense GEMM with the
-    same bytes and flops as the 32-RHS call: A (M x K, M*K = E) times
+    same bytes and flops as the 32-RHS call (E64 FP64 entries, or the E32
+    entries of the FP32 copy): A (M x K, M*K = E) times
     B (K x k), for two shapes of A in both storage orders.  A reference point
     for what a matrix-core GEMM of this arithmetic intensity reaches on the
     chip, not a bound."""
     import torch
     torch.backends.cuda.matmul.allow_tf32 = False
     out = {}
-    for dt, name in ((torch.float32, "fp32"), (torch.float64, "fp64")):
+    for dt, name, E in ((torch.float32, "fp32", E32), (torch.float64, "fp64", E64)):
         runs = {}
         for M in (32768, 150528):
             K = E // M
@@ -496,7 +497,10 @@ def bench_batching(args, cfg):
     t_build = time.perf_counter() - t0
     n, E, k = host.size, host.nb_entries(), cfg["nrhs"]
     log(f"c4 batch matrix: N={n} E={E} build {t_build:.1f}s")
-    dev = HipHmat(host, precisions=("fp64", "fp32"), device=0)
+    # the FP32 copy as in the c3 line: low-rank blocks recompressed to FP32 accuracy
+    tol32 = args.fp32_rank_tol
+    dev = HipHmat(host, precisions=("fp64", "fp32"), device=0, fp32_rank_tol=tol32)
+    E32 = int(sum(dev.entries["fp32"][:3]))
     rng = np.random.default_rng(1234)
     X = torch.from_numpy(rng.uniform(-1.0, 1.0, (n, k))).to("cuda:0")
     steps, warmup = args.steps, args.warmup
@@ -523,7 +527,8 @@ def bench_batching(args, cfg):
     vs_dense, _ = dense_check(host, dev, X[:, 0].cpu().numpy())
     log(f"vs dense operator {vs_dense}")
     flops = 2.0 * k * E
-    bytes_b = 4 * E + k * 8 * 2 * n      # FP32 factors once, X read and Y written in float64
+    flops32 = 2.0 * k * E32
+    bytes_b = 4 * E32 + k * 8 * 2 * n    # FP32 factors once, X read and Y written in float64
     bytes_b64 = 8 * E + k * 8 * 2 * n
     rhs_s = k / t_b32
     log(f"32-RHS fp32 batched {t_b32 * 1e3:.3f} ms, single fp32 {t_s32 * 1e3:.3f} ms, "
@@ -537,8 +542,9 @@ def bench_batching(args, cfg):
     t_mbuild = time.perf_counter() - t0
     sizes = [h.size for h in hosts]
     log(f"16 meshes: sides {sides}, DoF {sum(sizes)}, build {t_mbuild:.1f}s")
-    multi = HipHmat.many(hosts, precisions=("fp64", "fp32"), device=0)
-    singles = [HipHmat(h, precisions=("fp64", "fp32"), device=0) for h in hosts]
+    multi = HipHmat.many(hosts, precisions=("fp64", "fp32"), device=0, fp32_rank_tol=tol32)
+    singles = [HipHmat(h, precisions=("fp64", "fp32"), device=0, fp32_rank_tol=tol32)
+               for h in hosts]
     ntot = int(multi.row_offsets[-1])
     xm = torch.from_numpy(rng.uniform(-1.0, 1.0, ntot)).to("cuda:0")
     res = {}
@@ -568,7 +574,9 @@ def bench_batching(args, cfg):
     Xm = torch.from_numpy(rng.uniform(-1.0, 1.0, (ntot, k))).to("cuda:0")
     xparts = [Xm[multi.col_offsets[i]:multi.col_offsets[i + 1]] for i in range(len(hosts))]
     e_multi = sum(h.nb_entries() for h in hosts)
+    e_multi32 = int(sum(multi.entries["fp32"][:3]))
     for prec, es in (("fp32", 4), ("fp64", 8)):
+        e_m = e_multi32 if prec == "fp32" else e_multi
         if not args.no_autotune:
             from bigwham_dev_amd import autotune
             res[prec]["batched_tuning"] = autotune.tune_mfma(multi, k, precision=prec)
@@ -591,18 +599,18 @@ def bench_batching(args, cfg):
             "16_separate_ms": round(t_sb * 1e3, 4),
             "speedup_graph_vs_separate": round(t_sb / t_gb, 2),
             "rhs_per_s": round(k / t_gb, 1),
-            "hbm_GBs": round((es * e_multi + k * 8 * 2 * ntot) / t_gb / 1e9, 1),
-            "tflops": round(2.0 * k * e_multi / t_gb / 1e12, 2),
+            "hbm_GBs": round((es * e_m + k * 8 * 2 * ntot) / t_gb / 1e9, 1),
+            "tflops": round(2.0 * k * e_m / t_gb / 1e12, 2),
             "rel_diff_vs_separate": float(torch.linalg.matrix_norm(Yg - Ycat) /
                                           torch.linalg.matrix_norm(Ycat))}
         gb.close()
     log(f"16 meshes x 32 RHS: {({p: res[p]['batched_32rhs'] for p in res})}")
-    dense = dense_gemm_reference(E, k)
+    dense = dense_gemm_reference(E, E32, k)
     dense["fp32"]["dense_best_over_h_call"] = round(dense["fp32"]["best_ms"] / (t_b32 * 1e3), 3)
     dense["fp64"]["dense_best_over_h_call"] = round(dense["fp64"]["best_ms"] / (t_b64 * 1e3), 3)
     log(f"dense GEMM of the same size through the vendor BLAS: {dense}")
 
-    def roof(t, es, peak_tf, nbytes):
+    def roof(t, es, peak_tf, nbytes, flops):
         # attainable roof: the slower of the MFMA peak and HBM at 8 TB/s
         t_mfma = flops / (peak_tf * 1e12)
         t_hbm = nbytes / (HBM_PEAK_GBS * 1e9)
@@ -623,12 +631,14 @@ def bench_batching(args, cfg):
         "n_gpus": 1, "steps": steps, "warmup": warmup,
         "ms_per_step": round(t_b32 * 1e3, 4), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": cfg["workload"], "n_dof": n, "nb_entries": E, "nrhs": k},
-        "roofline": roof(t_b32, 4, 157.3, bytes_b),
+        "config": {"workload": cfg["workload"], "n_dof": n, "nb_entries": E, "nrhs": k,
+                   "fp32_rank_tolerance": tol32, "fp32_entries": E32,
+                   "fp32_entries_vs_fp64": round(E32 / E, 4)},
+        "roofline": roof(t_b32, 4, 157.3, bytes_b, flops32),
         "single_rhs": {"fp32_ms": round(t_s32 * 1e3, 4), "fp64_ms": round(t_s64 * 1e3, 4),
                        "batched_speedup_vs_32_fp32_calls": round(k * t_s32 / t_b32, 2)},
         "fp64_batched": {"ms_per_call": round(t_b64 * 1e3, 4), "rhs_per_s": round(k / t_b64, 1),
-                         "roofline": roof(t_b64, 8, 78.6, bytes_b64),
+                         "roofline": roof(t_b64, 8, 78.6, bytes_b64, flops),
                          "speedup_vs_32_fp64_calls": round(k * t_s64 / t_b64, 2)},
         "parity": {"batched_fp32_rel_err_vs_fp64": err, "batched_fp64_rel_err_vs_fp64": err64},
         "vs_dense_operator": vs_dense,

@@ -2116,7 +2116,7 @@ int create_impl(const bw_flat_hmat* h, long double expected_area, int precision_
   hd->fp32_rank_tol = g_fp32_rank_tol.load();
   if ((precision_mask & BW_PREC_FP32) && hd->fp32_rank_tol > 0.0) {
     std::string rmsg;
-    if ((rc = bwl::recompress_flat(h, hd->fp32_rank_tol, &rf, &rmsg)))
+    if ((rc = bwl::recompress_flat(h, hd->fp32_rank_tol, &rf, &rmsg, expected_area)))
       return cleanup(fail(rc, "create: recompression: " + rmsg));
     h32 = &rf.flat;
   }

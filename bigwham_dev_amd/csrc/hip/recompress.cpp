@@ -307,9 +307,10 @@ void recompress_block(int64_t m, int64_t n, int64_t r, const double* A, const do
 
 }  // namespace
 
-int recompress_flat(const bw_flat_hmat* h, double tol, RecompressedFlat* out, std::string* msg) {
+int recompress_flat(const bw_flat_hmat* h, double tol, RecompressedFlat* out, std::string* msg,
+                    long double expected_area) {
   std::string vmsg;
-  int rc = validate_flat(h, &vmsg);
+  int rc = validate_flat(h, &vmsg, expected_area);
   if (rc) {
     *msg = vmsg;
     return rc;

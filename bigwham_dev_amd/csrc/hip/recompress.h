@@ -23,6 +23,9 @@ struct RecompressedFlat {
 
 // Every low-rank block A B of h replaced by the shortest A' B' with
 // ||A B - A' B'||_F <= tol ||A B||_F (0 < tol < 1).
-int recompress_flat(const bw_flat_hmat* h, double tol, RecompressedFlat* out, std::string* msg);
+// expected_area: the point pairs the blocks must cover (-1: the whole
+// matrix; a block-diagonal composite covers only its diagonal blocks).
+int recompress_flat(const bw_flat_hmat* h, double tol, RecompressedFlat* out, std::string* msg,
+                    long double expected_area = -1.0L);
 
 }  // namespace bwl

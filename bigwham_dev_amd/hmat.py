@@ -212,10 +212,12 @@ class HipHmat:
         self.fp32_rank_tol = float(fp32_rank_tol)
 
     @classmethod
-    def many(cls, hosts, precisions=("fp64",), adjoint: bool = False, device: int = -1):
+    def many(cls, hosts, precisions=("fp64",), adjoint: bool = False, device: int = -1,
+             fp32_rank_tol: float = 0.0):
         """One operator for many independent meshes: diag(H_0, ..., H_{n-1}) on
         the concatenated vectors (bw_hip_create_multi).  ``row_offsets`` /
-        ``col_offsets`` give each mesh's slice of y and x."""
+        ``col_offsets`` give each mesh's slice of y and x.  fp32_rank_tol as
+        for the constructor."""
         self = cls.__new__(cls)
         mask, flags = cls._mask(precisions, adjoint)
         flats = [h.flat() for h in hosts]
@@ -223,11 +225,17 @@ class HipHmat:
         ro = np.zeros(len(flats) + 1, np.int64)
         co = np.zeros(len(flats) + 1, np.int64)
         out = C.c_void_p()
-        N.check_hip(N.hip().bw_hip_create_multi(arr, len(flats), mask, flags, int(device),
+        lib = N.hip()
+        N.check_hip(lib.bw_hip_set_fp32_rank_tolerance(float(fp32_rank_tol)))
+        try:
+            N.check_hip(lib.bw_hip_create_multi(arr, len(flats), mask, flags, int(device),
                                                 C.byref(out), N.iptr(ro), N.iptr(co)))
+        finally:
+            lib.bw_hip_set_fp32_rank_tolerance(0.0)
         self._host = list(hosts)
         self._finish(out, precisions, adjoint, device)
         self.row_offsets, self.col_offsets = ro, co
+        self.fp32_rank_tol = float(fp32_rank_tol)
         return self
 
     @classmethod

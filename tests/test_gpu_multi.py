@@ -54,6 +54,34 @@ def test_many_meshes_one_operator(gpu, meshes):
         assert np.array_equal(d[sl], O.diagonal(h.flat(), original=True))
 
 
+def test_many_meshes_recompressed_fp32(gpu, meshes):
+    """The composite's FP32 copy with its low-rank blocks recompressed
+    (fp32_rank_tol, as for one mesh): the blocks cover only the diagonal
+    meshes, the FP32 products (H x, H^T x, 32 right-hand sides) stay within
+    1e-5 of the oracle mesh by mesh, and the FP64 copy is untouched."""
+    import torch
+    dev = HipHmat.many(meshes, precisions=("fp64", "fp32"), adjoint=True, fp32_rank_tol=1e-7)
+    ref_dev = HipHmat.many(meshes, precisions=("fp64", "fp32"))
+    assert sum(dev.entries["fp32"][:3]) <= sum(ref_dev.entries["fp32"][:3])
+    off = dev.row_offsets
+    rng = np.random.default_rng(23)
+    x = rng.uniform(-1, 1, int(off[-1]))
+    xd = torch.from_numpy(x).to(gpu)
+    y64 = dev.matvec(xd).cpu().numpy()
+    y32 = dev.matvec(xd, precision="fp32").cpu().numpy()
+    yt32 = dev.rmatvec(xd, precision="fp32").cpu().numpy()
+    X = rng.uniform(-1, 1, (int(off[-1]), 32))
+    Y32 = dev.matvec_batched(torch.from_numpy(X).to(gpu), precision="fp32").cpu().numpy()
+    for m, h in enumerate(meshes):
+        sl = slice(off[m], off[m + 1])
+        ref = O.matvec_original(h.flat(), x[sl])
+        assert rel(y64[sl], ref) <= TOL64
+        assert rel(y32[sl], ref) <= TOL32
+        assert rel(yt32[sl], O.rmatvec_original(h.flat(), x[sl])) <= TOL32
+        for j in (0, 31):
+            assert rel(Y32[sl, j], O.matvec_original(h.flat(), X[sl, j])) <= TOL32
+
+
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_graph_replays_match_eager(gpu, meshes, precision):
     """Replays of the captured matvec, with eager matvecs and torch work in

@@ -77,6 +77,21 @@ def load_traffic(config):
     return out, doc.get("fp32_rank_tolerance", 0.0)
 
 
+def load_call_traffic(path, tol32):
+    """HBM bytes per batched FP32 call from a PMC summary (None if absent or
+    measured on an FP32 copy of another recompression tolerance)."""
+    import re
+    if not (ROOT / path).exists():
+        return None
+    m = re.search(r"```json\n(.*?)```", (ROOT / path).read_text(), re.S)
+    if not m:
+        return None
+    doc = json.loads(m.group(1))
+    if doc.get("fp32_rank_tolerance", 0.0) != tol32:
+        return None
+    return doc.get("traffic_bytes_per_call", {}).get("fp32")
+
+
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
@@ -634,7 +649,10 @@ def bench_batching(args, cfg):
         "config": {"workload": cfg["workload"], "n_dof": n, "nb_entries": E, "nrhs": k,
                    "fp32_rank_tolerance": tol32, "fp32_entries": E32,
                    "fp32_entries_vs_fp64": round(E32 / E, 4)},
-        "roofline": roof(t_b32, 4, 157.3, bytes_b, flops32),
+        "roofline": {**roof(t_b32, 4, 157.3, bytes_b, flops32),
+                     "traffic": load_call_traffic("profiles/hbm_traffic_3dr0_150k_dof_32rhs.md",
+                                                  tol32),
+                     "traffic_source": "profiles/hbm_traffic_3dr0_150k_dof_32rhs.md"},
         "single_rhs": {"fp32_ms": round(t_s32 * 1e3, 4), "fp64_ms": round(t_s64 * 1e3, 4),
                        "batched_speedup_vs_32_fp32_calls": round(k * t_s32 / t_b32, 2)},
         "fp64_batched": {"ms_per_call": round(t_b64 * 1e3, 4), "rhs_per_s": round(k / t_b64, 1),

@@ -141,6 +141,28 @@ def test_batched_fp32_mfma(gpu, cases, nrhs, original):
             assert rel(Y[:, k], ref) <= TOL32, (name, k)
 
 
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+@pytest.mark.parametrize("nrhs", [6, 32, 34])
+def test_batched_gather_paths_agree(gpu, cases, precision, nrhs):
+    """A row-major X (gathered four columns per thread with 16-byte loads)
+    and the same X stored column-major (gathered one element per thread) give
+    the same bits; nrhs 6 and 34 leave partial four-column groups."""
+    import torch
+    h = cases["3DR0"]
+    print(f"gather paths {precision} nrhs={nrhs}", flush=True)
+    dev = HipHmat(h, precisions=("fp64", "fp32"))
+    X = torch.from_numpy(np.random.default_rng(300 + nrhs).uniform(-1, 1, (h.size, nrhs))).to(gpu)
+    Xc = X.t().contiguous().t()  # column-major copy of the same values
+    assert X.stride() == (nrhs, 1) and Xc.stride() == (1, h.size)
+    Yr = dev.matvec_batched(X, precision=precision)
+    Yc = dev.matvec_batched(Xc, precision=precision)
+    assert torch.equal(Yr, Yc)
+    f = h.flat()
+    tol = TOL32 if precision == "fp32" else TOL64
+    for k in (0, nrhs - 1):
+        assert rel(Yr[:, k].cpu().numpy(), O.matvec_original(f, X[:, k].cpu().numpy())) <= tol
+
+
 @pytest.mark.parametrize("unroll", [4, 8, 16, (4, 16), (16, 8)])
 def test_unroll_depths(gpu, cases, unroll):
     """Every unroll depth the autotuner may pick, the same in both passes or

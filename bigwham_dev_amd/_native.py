@@ -69,13 +69,18 @@ def host() -> C.CDLL:
         lib.bwh_last_error.restype = C.c_char_p
         lib.bwh_create.argtypes = [C.c_char_p, _d, C.c_int64, _i64, C.c_int64, _d, C.c_int,
                                    C.c_int, C.c_int, C.c_int, C.POINTER(_vp)]
+        lib.bwh_create_rect.argtypes = [C.c_char_p, _d, C.c_int64, _i64, C.c_int64, _d,
+                                        C.c_int64, _i64, C.c_int64, _d, C.c_int, C.c_int,
+                                        C.c_int, C.c_int, C.POINTER(_vp)]
         lib.bwh_create_gaussian.argtypes = [C.c_int64, C.c_double, C.c_int, C.c_int,
                                             C.POINTER(_vp)]
         lib.bwh_build_pattern.argtypes = [_vp, C.c_int, C.c_double]
         lib.bwh_build_hmat.argtypes = [_vp, C.c_int, C.c_double, C.c_double]
         lib.bwh_flat.argtypes = [_vp, C.POINTER(FlatHmat)]
         lib.bwh_info.argtypes = [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), _i64]
+        lib.bwh_shape.argtypes = [_vp, _i64, _i64]
         lib.bwh_collocation_points.argtypes = [_vp, _d]
+        lib.bwh_receiver_collocation_points.argtypes = [_vp, _d]
         lib.bwh_element_normals.argtypes = [_vp, _d]
         lib.bwh_rotation_matrices.argtypes = [_vp, _d]
         lib.bwh_dense_block.argtypes = [_vp, C.c_int64, C.c_int64, C.c_int64, C.c_int64, _d]
@@ -183,8 +188,9 @@ def iptr(a: np.ndarray):
 
 
 # exported symbols each header declares (checked by the CPU tests)
-HOST_SYMBOLS = ["bwh_create", "bwh_create_gaussian", "bwh_build_pattern", "bwh_build_hmat",
-                "bwh_flat", "bwh_info", "bwh_collocation_points", "bwh_element_normals",
+HOST_SYMBOLS = ["bwh_create", "bwh_create_rect", "bwh_create_gaussian", "bwh_build_pattern",
+                "bwh_build_hmat", "bwh_flat", "bwh_info", "bwh_shape", "bwh_collocation_points",
+                "bwh_receiver_collocation_points", "bwh_element_normals",
                 "bwh_rotation_matrices", "bwh_dense_block", "bwh_max_aca_error",
                 "bwh_timings", "bwh_free", "bwh_last_error", "bwh_influence"]
 HIP_SYMBOLS = ["bw_hip_create", "bw_hip_matvec", "bw_hip_matvec_noalloc", "bw_hip_rmatvec", "bw_hip_matvec_batched",

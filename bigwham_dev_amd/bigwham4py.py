@@ -15,6 +15,12 @@ compatibility; there is no CPU matvec fallback in this package.
 Extras of this engine: `precision="fp32"` for FP32-factor matvecs,
 `matvec_torch(x)` on device tensors, `rmatvec` (H^T), and `solve()`
 (FP64 or mixed-precision GMRES on the device).
+
+BEMatrixRectangular mirrors bigwham4py.BEMatrixRectangular (bigwham4py.py:
+491-708): an operator from a source mesh to the collocation points of a
+receiver mesh, shape (N_receivers, N_sources).  The reference builds it for
+its CPU matvec only (bigwham_io.cc:310-314 aborts with useCuda); here its
+matvec, H^T and batched products run on the GPU like the square operator's.
 """
 from __future__ import annotations
 
@@ -28,6 +34,7 @@ from scipy.sparse.linalg import LinearOperator, spilu
 from .hmat import HipHmat, HostHmat, full_blocks
 
 kernels_id = ["2DS0-H", "3DT0-H", "3DR0-H", "3DR0-H-mode1"]
+kernels_id_rect = ["2DS0-2DS0-H", "3DT0-3DT0-H", "3DR0-3DR0-H", "3DR0-3DR0-H-mode1"]
 
 
 class BEMatrix(LinearOperator):
@@ -271,3 +278,141 @@ class BEMatrix(LinearOperator):
         if mixed:
             return solvers.solve_mixed(self.dev_, bt, tol=tol, restart=restart)
         return solvers.solve_fp64(self.dev_, bt, tol=tol, restart=restart)
+
+
+class BEMatrixRectangular(LinearOperator):
+    """Rectangular operator between two meshes (bigwham4py.py:491-708): the
+    traction at the receiver mesh's collocation points, in the receivers'
+    local frames, due to displacement discontinuities on the source mesh's
+    elements.  Same constructor arguments as the reference; `useCuda` is
+    accepted for signature compatibility (the matvec always runs on the GPU).
+    """
+
+    def __init__(self, kernel: str, coor_src: np.ndarray, conn_src: np.ndarray,
+                 coor_rec: np.ndarray, conn_rec: np.ndarray, properties: np.ndarray,
+                 max_leaf_size: int = 100, eta: float = 3.0, eps_aca: float = 1.0e-3,
+                 n_openMP_threads: int = 8, n_GPUs: int = 1, directly_build: bool = True,
+                 verbose: bool = True, homogeneous_size_pattern: bool = False,
+                 useCuda: bool = False, fixed_rank: int = -1, precisions=("fp64", "fp32"),
+                 adjoint: bool = True, device: int = -1):
+        if kernel not in kernels_id_rect:
+            raise ValueError(f"Invalid kernel : {kernel}, available kernels are : "
+                             f"[{', '.join(kernels_id_rect)}]")
+        if n_GPUs != 1:
+            raise ValueError("this engine drives one GPU per BEMatrixRectangular")
+        self.kernel_ = kernel
+        self.properties_ = np.asarray(properties, dtype=float)
+        self.max_leaf_size_ = int(max_leaf_size)
+        self.eta_ = float(eta)
+        self.eps_aca_ = float(eps_aca)
+        self.n_openMP_threads_ = n_openMP_threads
+        self.verbose_ = verbose
+        self._precisions = tuple(precisions)
+        self._adjoint = adjoint
+        self._device = device
+        self.H_ = HostHmat.from_meshes(kernel, coor_src, conn_src, coor_rec, conn_rec,
+                                       self.properties_, n_threads=n_openMP_threads,
+                                       homogeneous_size_pattern=homogeneous_size_pattern,
+                                       fixed_rank=fixed_rank)
+        self.dev_ = None
+        self.built_ = False
+        self.precision = "fp64"
+        self.shape_ = self.H_.shape
+        self.matvec_size_ = self.shape_[0]
+        if directly_build:
+            self._build()
+        else:
+            self.H_.build_pattern(self.max_leaf_size_, self.eta_)
+            super().__init__(float, self.shape_)
+
+    def _build(self):
+        if self.built_:
+            return
+        t0 = time.perf_counter()
+        self.H_.build(self.max_leaf_size_, self.eta_, self.eps_aca_)
+        self.dev_ = HipHmat(self.H_, precisions=self._precisions, adjoint=self._adjoint,
+                            device=self._device)
+        self.hmat_time_ = time.perf_counter() - t0
+        super().__init__(float, self.shape_)
+        self.built_ = True
+        if self.verbose_:
+            print(f"BEMatrixRectangular {self.kernel_}: {self.shape_[0]} x {self.shape_[1]}, "
+                  f"compression ratio {self.getCompression():.5f}, build {self.hmat_time_:.2f} s")
+
+    # ---- LinearOperator ---------------------------------------------------
+    def _matvec(self, v: np.ndarray) -> np.ndarray:
+        self._build()
+        return self.dev_.matvec(np.asarray(v, dtype=np.float64).ravel(), original=True,
+                                precision=self.precision)
+
+    def _rmatvec(self, v: np.ndarray) -> np.ndarray:
+        self._build()
+        return self.dev_.rmatvec(np.asarray(v, dtype=np.float64).ravel(), original=True,
+                                 precision=self.precision)
+
+    def matvec_torch(self, x, precision: str | None = None):
+        self._build()
+        return self.dev_.matvec(x, original=True, precision=precision or self.precision)
+
+    # ---- reference helpers -------------------------------------------------
+    def getCompression(self) -> float:
+        self._build()
+        return self.H_.compression_ratio()
+
+    def getPermutation(self) -> np.ndarray:
+        """Source (column) permutation."""
+        return self.H_.permutation()
+
+    def getPermutationReceivers(self) -> np.ndarray:
+        """Receiver (row) permutation."""
+        return self.H_.permutation_receivers()
+
+    def get_omp_threads(self) -> int:
+        """Host threads of the H-matrix construction (the matvec runs on the GPU)."""
+        n = self.n_openMP_threads_
+        return n if n and n > 0 else (os.cpu_count() or 1)
+
+    def getMeshCollocationPoints(self) -> np.ndarray:
+        """Collocation points of the receiver mesh, (n_receivers, dim), mesh
+        order (BigWhamIO::GetCollocationPoints reads the receiver mesh)."""
+        return self.H_.receiver_collocation_points()
+
+    def getCollocationPoints(self) -> np.ndarray:
+        """The receivers' collocation points in mesh order.  (The reference
+        scatters them through the source permutation, which fits only when
+        both meshes have as many points; the receivers are returned as the
+        mesh holds them.)"""
+        return self.H_.receiver_collocation_points()
+
+    def getSpatialDimension(self) -> int:
+        return self.H_.info()[1]
+
+    def _getPattern(self) -> np.ndarray:
+        """Rows (i_begin, j_begin, i_end, j_end, FLAG, entry size) as
+        BigWhamIO::GetHPattern, rows in receiver points and columns in source
+        points: FLAG 0 full rank, 1 low rank."""
+        fp, lp = self.H_.fr_pattern(), self.H_.lr_pattern()
+        rows = [np.stack([fp[1], fp[2], fp[3], fp[4], np.zeros_like(fp[1]),
+                          (fp[3] - fp[1]) * (fp[4] - fp[2])], 1),
+                np.stack([lp[1], lp[2], lp[3], lp[4], np.ones_like(lp[1]), lp[5]], 1)]
+        return np.concatenate(rows, 0)
+
+    def _getFullBlocks(self, original_order: bool = True) -> csc_matrix:
+        """Entries of the full-rank blocks as a sparse (N_receivers, N_sources)
+        matrix (bigwham4py.py:668-679)."""
+        self._build()
+        val, row, col = full_blocks(self.H_, original=original_order)
+        return csc_matrix((val, (row, col)), shape=self.shape_)
+
+    def get_max_error_ACA(self) -> float:
+        return self.H_.max_aca_error()
+
+    def getMatvecTime(self, N_matvec: int = 100) -> float:
+        """Seconds per matvec.  The input has shape[1] entries (the reference
+        sizes it with shape[0], which fits only square operators)."""
+        x = np.ones(self.shape[1])
+        self._matvec(x)
+        t0 = time.time()
+        for _ in range(N_matvec):
+            self._matvec(x)
+        return (time.time() - t0) / N_matvec

@@ -2,13 +2,16 @@
 //
 // Sequence and semantics follow the reference facade:
 //   BigWhamIO::BigWhamIO (square)        src/io/bigwham_io.cc:41-290
+//   BigWhamIO::BigWhamIO (rectangular)   src/io/bigwham_io.cc:292-484
 //   BigWhamIO::BuildPattern              src/io/bigwham_io.cc:487-512
 //   HRepresentationSquareMatrix          src/hmat/hierarchical_representation.h:40-86
+//   HRepresentationRectangularMatrix     src/hmat/hierarchical_representation.h:88-137
 //   BigWhamIO::BuildHierarchicalMatrix   src/io/bigwham_io.cc:635-704
 //   Hmat::buildFR / buildLR              src/hmat/hmatrix/hmat.cc:315-416
 // The block fill uses BieMatrixGenerator::set semantics
 // (src/hmat/bie_matrix_generator.h:93-141): column j1 of points is the source
-// element perm1[b1+j1], row j0 the receiver perm0[b0+j0], and the dof x dof
+// element perm1[b1+j1], row j0 the receiver perm0[b0+j0] (of the receiver
+// mesh, which is the source mesh for a square operator), and the dof x dof
 // kernel output lands at M(j0*dof + i, j1*dof + j) = st[j*dof + i].
 #include <algorithm>
 #include <atomic>
@@ -77,9 +80,10 @@ double now_s() {
 
 class BieGenerator final : public bwh::Generator {
  public:
-  BieGenerator(const std::vector<bwh::Element>* elts, const bwh::Kernel* k,
-               const std::vector<int64_t>* perm0, const std::vector<int64_t>* perm1)
-      : elts_(elts), k_(k), perm0_(perm0), perm1_(perm1), dof_(k->dof()) {}
+  BieGenerator(const std::vector<bwh::Element>* src, const std::vector<bwh::Element>* rec,
+               const bwh::Kernel* k, const std::vector<int64_t>* perm0,
+               const std::vector<int64_t>* perm1)
+      : src_(src), rec_(rec), k_(k), perm0_(perm0), perm1_(perm1), dof_(k->dof()) {}
   int dof() const override { return dof_; }
   int64_t rows_pts() const override { return static_cast<int64_t>(perm0_->size()); }
   int64_t cols_pts() const override { return static_cast<int64_t>(perm1_->size()); }
@@ -87,9 +91,9 @@ class BieGenerator final : public bwh::Generator {
            int64_t ldm) const override {
     double st[9];
     for (int64_t j1 = 0; j1 < n; ++j1) {
-      const bwh::Element& src = (*elts_)[(*perm1_)[b1 + j1]];
+      const bwh::Element& src = (*src_)[(*perm1_)[b1 + j1]];
       for (int64_t j0 = 0; j0 < m; ++j0) {
-        const bwh::Element& rec = (*elts_)[(*perm0_)[b0 + j0]];
+        const bwh::Element& rec = (*rec_)[(*perm0_)[b0 + j0]];
         k_->influence(src, rec, st);
         for (int j = 0; j < dof_; ++j)
           for (int i = 0; i < dof_; ++i)
@@ -99,7 +103,8 @@ class BieGenerator final : public bwh::Generator {
   }
 
  private:
-  const std::vector<bwh::Element>* elts_;
+  const std::vector<bwh::Element>* src_;
+  const std::vector<bwh::Element>* rec_;
   const bwh::Kernel* k_;
   const std::vector<int64_t>* perm0_;
   const std::vector<int64_t>* perm1_;
@@ -133,6 +138,33 @@ class GaussianGenerator final : public bwh::Generator {
   double alpha_;
 };
 
+// Elements and collocation points of one mesh (coor: n_nodes x dim, conn:
+// n_elts x nvert, both row-major), as CreateMeshFromVect (core/mesh.h).
+int make_mesh(int dim, int nvert, const double* coor, int64_t n_nodes, const int64_t* conn,
+              int64_t n_elts, std::vector<bwh::Element>& elts, std::vector<double>& colloc) {
+  for (int64_t i = 0; i < n_elts * nvert; ++i)
+    if (conn[i] < 0 || conn[i] >= n_nodes)
+      return fail(BW_ESHAPE, "connectivity index out of range");
+  elts.resize(n_elts);
+  colloc.resize(n_elts * dim);
+  for (int64_t e = 0; e < n_elts; ++e) {
+    bwh::Element& el = elts[e];
+    el.id = e;
+    if (dim == 2) {
+      double x0[2] = {coor[conn[2 * e] * 2], coor[conn[2 * e] * 2 + 1]};
+      double x1[2] = {coor[conn[2 * e + 1] * 2], coor[conn[2 * e + 1] * 2 + 1]};
+      bwh::set_segment(el, x0, x1);
+    } else {
+      double xv[4][3];
+      for (int v = 0; v < nvert; ++v)
+        for (int j = 0; j < 3; ++j) xv[v][j] = coor[conn[e * nvert + v] * 3 + j];
+      bwh::set_polygon(el, nvert, xv);
+    }
+    for (int j = 0; j < dim; ++j) colloc[e * dim + j] = el.colloc[j];
+  }
+  return BW_OK;
+}
+
 }  // namespace
 
 struct bwh_hmat {
@@ -149,6 +181,11 @@ struct bwh_hmat {
   std::vector<bwh::Element> elts;
   std::vector<double> colloc;  // n x dim, original order
   std::unique_ptr<bwh::Kernel> kernel;
+  // rectangular operator: the receiver mesh (rows); elts / colloc above are
+  // then the source mesh (columns)
+  bool rect = false;
+  std::vector<bwh::Element> relts;
+  std::vector<double> rcolloc;
   // Gaussian test operator
   bool gaussian = false;
   int64_t gauss_n = 0;
@@ -157,7 +194,8 @@ struct bwh_hmat {
   bool pattern_built = false;
   int leaf = 0;
   double eta = 0.0;
-  std::vector<int64_t> perm;  // square: perm0 == perm1
+  std::vector<int64_t> perm;   // columns (sources); square: perm0 == perm1
+  std::vector<int64_t> rperm;  // rows (receivers) of a rectangular operator
   std::vector<int64_t> fr_pattern, lr_pattern;  // 6 x n col-major
   // blocks
   bool built = false;
@@ -170,9 +208,14 @@ struct bwh_hmat {
   int64_t npts() const {
     return gaussian ? gauss_n : static_cast<int64_t>(colloc.size()) / std::max(dim, 1);
   }
+  int64_t row_pts() const {
+    return rect ? static_cast<int64_t>(rcolloc.size()) / std::max(dim, 1) : npts();
+  }
+  const std::vector<int64_t>& perm0() const { return rect ? rperm : perm; }
   std::unique_ptr<bwh::Generator> generator() const {
     if (gaussian) return std::make_unique<GaussianGenerator>(gauss_n, gauss_alpha);
-    return std::make_unique<BieGenerator>(&elts, kernel.get(), &perm, &perm);
+    return std::make_unique<BieGenerator>(&elts, rect ? &relts : &elts, kernel.get(), &perm0(),
+                                          &perm);
   }
 };
 
@@ -197,27 +240,52 @@ int bwh_create(const char* kernel, const double* coor, int64_t n_nodes,
   h->fixed_rank = fixed_rank;
   h->coor.assign(coor, coor + n_nodes * h->dim);
   h->conn.assign(conn, conn + n_elts * h->nvert);
-  for (int64_t i = 0; i < n_elts * h->nvert; ++i)
-    if (h->conn[i] < 0 || h->conn[i] >= n_nodes)
-      return fail(BW_ESHAPE, "bwh_create: connectivity index out of range");
-  h->elts.resize(n_elts);
-  h->colloc.resize(n_elts * h->dim);
-  for (int64_t e = 0; e < n_elts; ++e) {
-    bwh::Element& el = h->elts[e];
-    el.id = e;
-    if (h->dim == 2) {
-      double x0[2] = {coor[h->conn[2 * e] * 2], coor[h->conn[2 * e] * 2 + 1]};
-      double x1[2] = {coor[h->conn[2 * e + 1] * 2], coor[h->conn[2 * e + 1] * 2 + 1]};
-      bwh::set_segment(el, x0, x1);
-    } else {
-      double xv[4][3];
-      for (int v = 0; v < h->nvert; ++v)
-        for (int j = 0; j < 3; ++j) xv[v][j] = coor[h->conn[e * h->nvert + v] * 3 + j];
-      bwh::set_polygon(el, h->nvert, xv);
-    }
-    for (int j = 0; j < h->dim; ++j) h->colloc[e * h->dim + j] = el.colloc[j];
-  }
+  if (make_mesh(h->dim, h->nvert, coor, n_nodes, conn, n_elts, h->elts, h->colloc) != BW_OK)
+    return fail(BW_ESHAPE, "bwh_create: " + g_last_error);
   h->kernel->prepare(h->elts);
+  *out = h.release();
+  return BW_OK;
+}
+
+int bwh_create_rect(const char* kernel, const double* coor_src, int64_t n_nodes_src,
+                    const int64_t* conn_src, int64_t n_elts_src, const double* coor_rec,
+                    int64_t n_nodes_rec, const int64_t* conn_rec, int64_t n_elts_rec,
+                    const double* props, int n_props, int n_threads,
+                    int homogeneous_size_pattern, int fixed_rank, bwh_hmat** out) {
+  if (!kernel || !coor_src || !conn_src || !coor_rec || !conn_rec || !props || !out ||
+      n_nodes_src <= 0 || n_elts_src <= 0 || n_nodes_rec <= 0 || n_elts_rec <= 0)
+    return fail(BW_EINVAL, "bwh_create_rect: null argument or empty mesh");
+  // "<elt>-<elt>-H<suffix>" with the same element on both sides names the
+  // square kernel "<elt>-H<suffix>" (bigwham_io.cc:345-475)
+  static const char* const names[][2] = {{"2DS0-2DS0-H", "2DS0-H"},
+                                         {"3DT0-3DT0-H", "3DT0-H"},
+                                         {"3DR0-3DR0-H", "3DR0-H"},
+                                         {"3DR0-3DR0-H-mode1", "3DR0-H-mode1"}};
+  const char* base = nullptr;
+  for (const auto& nm : names)
+    if (std::strcmp(kernel, nm[0]) == 0) base = nm[1];
+  if (!base)
+    return fail(BW_EINVAL, std::string("bwh_create_rect: unknown kernel '") + kernel +
+                               "' (2DS0-2DS0-H, 3DT0-3DT0-H, 3DR0-3DR0-H, 3DR0-3DR0-H-mode1)");
+  auto h = std::make_unique<bwh_hmat>();
+  h->kernel_name = kernel;
+  h->rect = true;
+  std::string err;
+  h->kernel = bwh::make_kernel(base, props, n_props, &h->dim, &h->nvert, &err);
+  if (!h->kernel) return fail(BW_EINVAL, "bwh_create_rect: " + err);
+  h->dof = h->kernel->dof();
+  h->nthreads = resolve_threads(n_threads);
+  h->homogeneous = homogeneous_size_pattern != 0;
+  h->fixed_rank = fixed_rank;
+  h->coor.assign(coor_src, coor_src + n_nodes_src * h->dim);
+  h->conn.assign(conn_src, conn_src + n_elts_src * h->nvert);
+  if (make_mesh(h->dim, h->nvert, coor_src, n_nodes_src, conn_src, n_elts_src, h->elts,
+                h->colloc) != BW_OK)
+    return fail(BW_ESHAPE, "bwh_create_rect: source mesh: " + g_last_error);
+  if (make_mesh(h->dim, h->nvert, coor_rec, n_nodes_rec, conn_rec, n_elts_rec, h->relts,
+                h->rcolloc) != BW_OK)
+    return fail(BW_ESHAPE, "bwh_create_rect: receiver mesh: " + g_last_error);
+  h->kernel->prepare(h->elts);  // source-side precomputation
   *out = h.release();
   return BW_OK;
 }
@@ -246,7 +314,15 @@ int bwh_build_pattern(bwh_hmat* h, int max_leaf_size, double eta) {
   std::vector<double> pts = h->colloc;  // permuted in place by the clustering
   bwh::ClusterTree ct = bwh::build_cluster_tree(max_leaf_size, pts, h->dim, h->homogeneous);
   std::vector<bwh::BlockEntry> fr, lr;
-  bwh::build_block_pattern(pts, h->dim, ct, eta, h->homogeneous, h->fixed_rank, fr, lr);
+  if (h->rect) {
+    // sources cluster first, then receivers (hierarchical_representation.h:104-118)
+    std::vector<double> rpts = h->rcolloc;
+    bwh::ClusterTree rt = bwh::build_cluster_tree(max_leaf_size, rpts, h->dim, h->homogeneous);
+    bwh::build_block_pattern_rect(rpts, rt, pts, ct, h->dim, eta, h->homogeneous, fr, lr);
+    h->rperm = rt.perm;
+  } else {
+    bwh::build_block_pattern(pts, h->dim, ct, eta, h->homogeneous, h->fixed_rank, fr, lr);
+  }
   h->perm = ct.perm;
   h->fr_pattern.resize(6 * fr.size());
   h->lr_pattern.resize(6 * lr.size());
@@ -351,9 +427,9 @@ int bwh_flat(const bwh_hmat* h, bw_flat_hmat* out) {
   // (block pointer arrays are null and low-rank ranks read 1).
   std::memset(out, 0, sizeof(*out));
   out->dof = h->dof;
-  out->n_row_pts = h->npts();
+  out->n_row_pts = h->row_pts();
   out->n_col_pts = h->npts();
-  out->perm0 = h->perm.data();
+  out->perm0 = h->perm0().data();
   out->perm1 = h->perm.data();
   out->n_fr = static_cast<int64_t>(h->fr_pattern.size() / 6);
   out->fr_pattern = h->fr_pattern.data();
@@ -373,9 +449,23 @@ int bwh_info(const bwh_hmat* h, int* dof, int* dim, int64_t* n_colloc) {
   return BW_OK;
 }
 
+int bwh_shape(const bwh_hmat* h, int64_t* n_row_pts, int64_t* n_col_pts) {
+  if (!h) return fail(BW_EINVAL, "bwh_shape: null handle");
+  if (n_row_pts) *n_row_pts = h->row_pts();
+  if (n_col_pts) *n_col_pts = h->npts();
+  return BW_OK;
+}
+
 int bwh_collocation_points(const bwh_hmat* h, double* out) {
   if (!h || !out) return fail(BW_EINVAL, "bwh_collocation_points: null argument");
   std::memcpy(out, h->colloc.data(), h->colloc.size() * sizeof(double));
+  return BW_OK;
+}
+
+int bwh_receiver_collocation_points(const bwh_hmat* h, double* out) {
+  if (!h || !out) return fail(BW_EINVAL, "bwh_receiver_collocation_points: null argument");
+  const std::vector<double>& c = h->rect ? h->rcolloc : h->colloc;
+  std::memcpy(out, c.data(), c.size() * sizeof(double));
   return BW_OK;
 }
 
@@ -401,8 +491,7 @@ int bwh_dense_block(const bwh_hmat* h, int64_t b0, int64_t b1, int64_t m, int64_
                     double* out) {
   if (!h || !out) return fail(BW_EINVAL, "bwh_dense_block: null argument");
   if (!h->pattern_built) return fail(BW_ESTATE, "bwh_dense_block: pattern not built");
-  const int64_t np = h->npts();
-  if (b0 < 0 || b1 < 0 || m < 0 || n < 0 || b0 + m > np || b1 + n > np)
+  if (b0 < 0 || b1 < 0 || m < 0 || n < 0 || b0 + m > h->row_pts() || b1 + n > h->npts())
     return fail(BW_ESHAPE, "bwh_dense_block: block outside the matrix");
   auto gen = h->generator();
   const int dof = h->dof;
@@ -431,10 +520,11 @@ int bwh_timings(const bwh_hmat* h, double* pattern_s, double* hmat_s) {
 int bwh_influence(const bwh_hmat* h, int64_t e_src, int64_t e_rec, double* out) {
   if (!h || !out) return fail(BW_EINVAL, "bwh_influence: null argument");
   if (h->gaussian) return fail(BW_ENOTSUP, "bwh_influence: no mesh");
-  const int64_t ne = static_cast<int64_t>(h->elts.size());
-  if (e_src < 0 || e_rec < 0 || e_src >= ne || e_rec >= ne)
+  const std::vector<bwh::Element>& rec = h->rect ? h->relts : h->elts;
+  if (e_src < 0 || e_rec < 0 || e_src >= static_cast<int64_t>(h->elts.size()) ||
+      e_rec >= static_cast<int64_t>(rec.size()))
     return fail(BW_ESHAPE, "bwh_influence: element index out of range");
-  h->kernel->influence(h->elts[e_src], h->elts[e_rec], out);
+  h->kernel->influence(h->elts[e_src], rec[e_rec], out);
   return BW_OK;
 }
 

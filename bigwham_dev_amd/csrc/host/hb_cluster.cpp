@@ -260,6 +260,57 @@ ClusterTree build_cluster_tree(int64_t leaf_size, std::vector<double>& pts,
 
 namespace {
 
+// Bounding box of every cluster node (dim mins then dim maxs per node), the
+// boxes that diameter() and distance() of cluster.h:121-147 measure.
+void node_bboxes(const std::vector<double>& pts, int dim, const ClusterTree& tree, int32_t s,
+                 std::vector<double>& bbox) {
+  const ClusterNode& nd = tree.nodes[s];
+  double* bb = &bbox[static_cast<size_t>(s) * 2 * dim];
+  if (tree.has_children(s)) {
+    node_bboxes(pts, dim, tree, nd.child[0], bbox);
+    node_bboxes(pts, dim, tree, nd.child[1], bbox);
+    const double* b0 = &bbox[static_cast<size_t>(nd.child[0]) * 2 * dim];
+    const double* b1 = &bbox[static_cast<size_t>(nd.child[1]) * 2 * dim];
+    for (int d = 0; d < dim; ++d) {
+      bb[d] = std::min(b0[d], b1[d]);
+      bb[dim + d] = std::max(b0[dim + d], b1[dim + d]);
+    }
+  } else {
+    for (int d = 0; d < dim; ++d) {
+      double mn = std::numeric_limits<double>::max();
+      double mx = -std::numeric_limits<double>::max();
+      for (int64_t i = nd.begin; i < nd.end; ++i) {
+        mn = std::min(mn, pts[i * dim + d]);
+        mx = std::max(mx, pts[i * dim + d]);
+      }
+      bb[d] = mn;
+      bb[dim + d] = mx;
+    }
+  }
+}
+
+std::vector<double> tree_bboxes(const std::vector<double>& pts, int dim, const ClusterTree& tree) {
+  std::vector<double> bbox(tree.nodes.size() * 2 * static_cast<size_t>(dim), 0.0);
+  node_bboxes(pts, dim, tree, 0, bbox);
+  return bbox;
+}
+
+// cluster.h:148-173: diameter = |bbox diagonal|, distance between the boxes,
+// admissible iff max(diam0, diam1) <= eta * dist.
+bool boxes_admissible(const double* ba, const double* bb, int dim, double eta) {
+  double da = 0.0, db = 0.0, dist = 0.0;
+  for (int d = 0; d < dim; ++d) {
+    const double wa = ba[dim + d] - ba[d];
+    const double wb = bb[dim + d] - bb[d];
+    da += wa * wa;
+    db += wb * wb;
+    const double g1 = std::max(0.0, bb[d] - ba[dim + d]);
+    const double g2 = std::max(0.0, ba[d] - bb[dim + d]);
+    dist += g1 * g1 + g2 * g2;
+  }
+  return std::max(std::sqrt(da), std::sqrt(db)) <= eta * std::sqrt(dist);
+}
+
 struct BlockBuilder {
   const std::vector<double>& pts;
   int dim;
@@ -271,54 +322,13 @@ struct BlockBuilder {
   std::vector<BlockEntry>* lr;
   int64_t next_spot = 1;
 
-  void compute_bbox(int32_t s) {
-    const ClusterNode& nd = tree.nodes[s];
-    double* bb = &bbox[static_cast<size_t>(s) * 2 * dim];
-    if (tree.has_children(s)) {
-      compute_bbox(nd.child[0]);
-      compute_bbox(nd.child[1]);
-      const double* b0 = &bbox[static_cast<size_t>(nd.child[0]) * 2 * dim];
-      const double* b1 = &bbox[static_cast<size_t>(nd.child[1]) * 2 * dim];
-      for (int d = 0; d < dim; ++d) {
-        bb[d] = std::min(b0[d], b1[d]);
-        bb[dim + d] = std::max(b0[dim + d], b1[dim + d]);
-      }
-    } else {
-      for (int d = 0; d < dim; ++d) {
-        double mn = std::numeric_limits<double>::max();
-        double mx = -std::numeric_limits<double>::max();
-        for (int64_t i = nd.begin; i < nd.end; ++i) {
-          mn = std::min(mn, pts[i * dim + d]);
-          mx = std::max(mx, pts[i * dim + d]);
-        }
-        bb[d] = mn;
-        bb[dim + d] = mx;
-      }
-    }
-  }
-
-  // cluster.h:148-160: diameter = |bbox diagonal|, distance between boxes,
-  // admissible iff max(diam0, diam1) <= eta * dist (never for identical ranges).
+  // never admissible for identical ranges (the diagonal blocks)
   bool admissible(int32_t s0, int32_t s1) const {
     const ClusterNode& a = tree.nodes[s0];
     const ClusterNode& b = tree.nodes[s1];
     if (a.begin == b.begin && a.end == b.end) return false;
-    const double* ba = &bbox[static_cast<size_t>(s0) * 2 * dim];
-    const double* bb = &bbox[static_cast<size_t>(s1) * 2 * dim];
-    double da = 0.0, db = 0.0, dist = 0.0;
-    for (int d = 0; d < dim; ++d) {
-      const double wa = ba[dim + d] - ba[d];
-      const double wb = bb[dim + d] - bb[d];
-      da += wa * wa;
-      db += wb * wb;
-      const double g1 = std::max(0.0, bb[d] - ba[dim + d]);
-      const double g2 = std::max(0.0, ba[d] - bb[dim + d]);
-      dist += g1 * g1 + g2 * g2;
-    }
-    da = std::sqrt(da);
-    db = std::sqrt(db);
-    dist = std::sqrt(dist);
-    return std::max(da, db) <= eta * dist;
+    return boxes_admissible(&bbox[static_cast<size_t>(s0) * 2 * dim],
+                            &bbox[static_cast<size_t>(s1) * 2 * dim], dim, eta);
   }
 
   void emit(int64_t spot, int32_t s0, int32_t s1, bool low_rank) {
@@ -398,15 +408,91 @@ struct BlockBuilder {
   }
 };
 
+// Receivers (rows, tree 0) against sources (columns, tree 1): the block
+// tree I x J of a rectangular operator (cluster.cc:213-374).  Children are
+// visited, and spots numbered, in the same order as for I x I.
+struct RectBlockBuilder {
+  const ClusterTree& t0;
+  const ClusterTree& t1;
+  int dim;
+  double eta;
+  std::vector<double> bb0, bb1;
+  std::vector<BlockEntry>* fr;
+  std::vector<BlockEntry>* lr;
+  int64_t next_spot = 1;
+
+  // isAdmissible for two point sets (cluster.h:161-173): identical ranges
+  // are not excluded, the two sets being different.
+  bool admissible(int32_t s0, int32_t s1) const {
+    return boxes_admissible(&bb0[static_cast<size_t>(s0) * 2 * dim],
+                            &bb1[static_cast<size_t>(s1) * 2 * dim], dim, eta);
+  }
+  void emit(int64_t spot, int32_t s0, int32_t s1, bool low_rank) {
+    const ClusterNode& a = t0.nodes[s0];
+    const ClusterNode& b = t1.nodes[s1];
+    (low_rank ? lr : fr)->push_back(BlockEntry{spot, a.begin, b.begin, a.end, b.end, low_rank});
+  }
+  void four(int32_t s0, int32_t s1, bool homog) {
+    const int32_t a0 = t0.nodes[s0].child[0], a1 = t0.nodes[s0].child[1];
+    const int32_t c0 = t1.nodes[s1].child[0], c1 = t1.nodes[s1].child[1];
+    const int32_t pr[4][2] = {{a0, c0}, {a1, c0}, {a0, c1}, {a1, c1}};
+    for (const auto& q : pr) {
+      const int64_t c = next_spot++;
+      homog ? rec_homog(c, q[0], q[1]) : rec_plain(c, q[0], q[1]);
+    }
+  }
+
+  // hmatrixTreeIxJ_rec (cluster.cc:233-279)
+  void rec_plain(int64_t spot, int32_t s0, int32_t s1) {
+    if (admissible(s0, s1)) return emit(spot, s0, s1, true);
+    if (t0.has_children(s0) && t1.has_children(s1)) return four(s0, s1, false);
+    emit(spot, s0, s1, false);
+  }
+
+  // hmatrixTreeIxJ_rec_size_conservative (cluster.cc:281-374): low rank only
+  // for admissible blocks with as many rows as columns of points; otherwise
+  // whichever side has children is split, the other kept whole.
+  void rec_homog(int64_t spot, int32_t s0, int32_t s1) {
+    const int64_t n0 = t0.nodes[s0].end - t0.nodes[s0].begin;
+    const int64_t n1 = t1.nodes[s1].end - t1.nodes[s1].begin;
+    if (n0 == n1 && admissible(s0, s1)) return emit(spot, s0, s1, true);
+    const bool h0 = t0.has_children(s0), h1 = t1.has_children(s1);
+    if (h0 && h1) return four(s0, s1, true);
+    if (h0 || h1) {
+      for (int i = 0; i < 2; ++i) {
+        const int64_t c = next_spot++;
+        if (h0)
+          rec_homog(c, t0.nodes[s0].child[i], s1);
+        else
+          rec_homog(c, s0, t1.nodes[s1].child[i]);
+      }
+      return;
+    }
+    emit(spot, s0, s1, false);
+  }
+};
+
 }  // namespace
 
 void build_block_pattern(const std::vector<double>& pts, int dim,
                          const ClusterTree& tree, double eta, bool homogeneous,
                          int fixed_rank, std::vector<BlockEntry>& fr,
                          std::vector<BlockEntry>& lr) {
-  BlockBuilder bb{pts, dim, tree, eta, fixed_rank, {}, &fr, &lr, 1};
-  bb.bbox.assign(tree.nodes.size() * 2 * static_cast<size_t>(dim), 0.0);
-  bb.compute_bbox(0);
+  BlockBuilder bb{pts, dim, tree, eta, fixed_rank, tree_bboxes(pts, dim, tree), &fr, &lr, 1};
+  fr.clear();
+  lr.clear();
+  if (homogeneous)
+    bb.rec_homog(0, 0, 0);
+  else
+    bb.rec_plain(0, 0, 0);
+}
+
+void build_block_pattern_rect(const std::vector<double>& pts0, const ClusterTree& tree0,
+                              const std::vector<double>& pts1, const ClusterTree& tree1,
+                              int dim, double eta, bool homogeneous,
+                              std::vector<BlockEntry>& fr, std::vector<BlockEntry>& lr) {
+  RectBlockBuilder bb{tree0, tree1, dim, eta, tree_bboxes(pts0, dim, tree0),
+                      tree_bboxes(pts1, dim, tree1), &fr, &lr, 1};
   fr.clear();
   lr.clear();
   if (homogeneous)

@@ -116,6 +116,14 @@ void build_block_pattern(const std::vector<double>& pts, int dim,
                          int fixed_rank, std::vector<BlockEntry>& fr,
                          std::vector<BlockEntry>& lr);
 
+// Block-cluster tree I x J of a rectangular operator: receivers (rows,
+// pts0 / tree0) against sources (columns, pts1 / tree1), with the pattern in
+// the same traversal order (hmatrixTreeIxJ, src/hmat/cluster/cluster.cc:213-374).
+void build_block_pattern_rect(const std::vector<double>& pts0, const ClusterTree& tree0,
+                              const std::vector<double>& pts1, const ClusterTree& tree1,
+                              int dim, double eta, bool homogeneous,
+                              std::vector<BlockEntry>& fr, std::vector<BlockEntry>& lr);
+
 // ---------------------------------------------------------------------------
 // Adaptive cross approximation with p x p pivots.
 struct LowRankBlock {

@@ -2,8 +2,9 @@
 
 HostHmat  — host-side construction (libbwhost.so): mesh, kernel, cluster tree,
             pattern, full-rank blocks and ACA low-rank blocks; mirrors
-            BigWhamIO's constructor / BuildPattern / BuildHierarchicalMatrix
-            (src/io/bigwham_io.cc:41-290, :487-512, :635-704).
+            BigWhamIO's constructors (square and rectangular) / BuildPattern /
+            BuildHierarchicalMatrix (src/io/bigwham_io.cc:41-290, :292-484,
+            :487-512, :635-704).
 HipHmat   — the MI355X engine (libbwhip.so): device-resident layout and the
             matvec / adjoint / batched entry points (drop-in for
             Hmat::matvec / matvecOriginal, src/hmat/hmatrix/hmat.cc:472-715,
@@ -39,6 +40,28 @@ class HostHmat:
                                          props.size, int(n_threads),
                                          int(bool(homogeneous_size_pattern)), int(fixed_rank),
                                          C.byref(out)))
+        return cls(out.value)
+
+    @classmethod
+    def from_meshes(cls, kernel: str, coor_src, conn_src, coor_rec, conn_rec, properties,
+                    n_threads: int = -1, homogeneous_size_pattern: bool = False,
+                    fixed_rank: int = -1) -> "HostHmat":
+        """Rectangular operator from the source mesh (columns) to the receiver
+        mesh's collocation points (rows); kernel "2DS0-2DS0-H", "3DT0-3DT0-H",
+        "3DR0-3DR0-H" or "3DR0-3DR0-H-mode1" (BigWhamIO rectangular
+        constructor, src/io/bigwham_io.cc:292-484)."""
+        cs = np.ascontiguousarray(coor_src, dtype=np.float64)
+        ks = np.ascontiguousarray(conn_src, dtype=np.int64)
+        cr = np.ascontiguousarray(coor_rec, dtype=np.float64)
+        kr = np.ascontiguousarray(conn_rec, dtype=np.int64)
+        props = np.ascontiguousarray(np.asarray(properties, dtype=np.float64).ravel())
+        out = C.c_void_p()
+        N.check_host(N.host().bwh_create_rect(kernel.encode(), N.dptr(cs), cs.shape[0],
+                                              N.iptr(ks), ks.shape[0], N.dptr(cr), cr.shape[0],
+                                              N.iptr(kr), kr.shape[0], N.dptr(props),
+                                              props.size, int(n_threads),
+                                              int(bool(homogeneous_size_pattern)),
+                                              int(fixed_rank), C.byref(out)))
         return cls(out.value)
 
     @classmethod
@@ -89,10 +112,32 @@ class HostHmat:
         dof, _, n = self.info()
         return dof * n
 
+    def shape_points(self):
+        """(receiver points, source points): equal for a square operator."""
+        r, c = C.c_int64(), C.c_int64()
+        N.check_host(N.host().bwh_shape(self._h, C.byref(r), C.byref(c)))
+        return r.value, c.value
+
+    @property
+    def shape(self):
+        """(rows, columns) of the operator in DoFs."""
+        d = self.dof
+        r, c = self.shape_points()
+        return d * r, d * c
+
     def collocation_points(self) -> np.ndarray:
         dof, dim, n = self.info()
         out = np.empty((n, dim))
         N.check_host(N.host().bwh_collocation_points(self._h, N.dptr(out)))
+        return out
+
+    def receiver_collocation_points(self) -> np.ndarray:
+        """Collocation points of the receiver mesh (the mesh's own for a square
+        operator), original order."""
+        _, dim, _ = self.info()
+        r, _ = self.shape_points()
+        out = np.empty((r, dim))
+        N.check_host(N.host().bwh_receiver_collocation_points(self._h, N.dptr(out)))
         return out
 
     def element_normals(self) -> np.ndarray:
@@ -113,6 +158,12 @@ class HostHmat:
     def permutation(self) -> np.ndarray:
         f = self.flat()
         return np.ctypeslib.as_array(f.perm1, shape=(f.n_col_pts,)).copy()
+
+    def permutation_receivers(self) -> np.ndarray:
+        """Row (receiver) permutation, perm0; the same array as permutation()
+        for a square operator."""
+        f = self.flat()
+        return np.ctypeslib.as_array(f.perm0, shape=(f.n_row_pts,)).copy()
 
     def fr_pattern(self) -> np.ndarray:
         f = self.flat()
@@ -152,8 +203,9 @@ class HostHmat:
         return e
 
     def compression_ratio(self) -> float:
-        n = self.size
-        return self.nb_entries() / float(n * n)
+        """nbOfEntries / (rows x columns) (Hmat::compressionRatio, hmat.cc:464-467)."""
+        r, c = self.shape
+        return self.nb_entries() / float(r * c)
 
     def dense_block(self, b0: int, b1: int, m: int, n: int) -> np.ndarray:
         d = self.dof
@@ -163,14 +215,16 @@ class HostHmat:
 
     def dense(self, original: bool = False) -> np.ndarray:
         """Exact operator (toArray2D, matrix_generator.h:35-40), permuted or original order."""
-        dof, _, n = self.info()
-        M = self.dense_block(0, 0, n, n)
+        dof = self.dof
+        r, c = self.shape_points()
+        M = self.dense_block(0, 0, r, c)
         if not original:
             return M
-        perm = self.permutation()
-        idx = (dof * perm[:, None] + np.arange(dof)[None, :]).ravel()
+        ar = np.arange(dof)[None, :]
+        ri = (dof * self.permutation_receivers()[:, None] + ar).ravel()
+        ci = (dof * self.permutation()[:, None] + ar).ravel()
         out = np.empty_like(M)
-        out[np.ix_(idx, idx)] = M
+        out[np.ix_(ri, ci)] = M
         return out
 
     def influence(self, e_src: int, e_rec: int) -> np.ndarray:

@@ -1,5 +1,6 @@
 // Host code under AddressSanitizer + UndefinedBehaviorSanitizer: H-matrix
-// construction (Gaussian, 3DR0-H, 3DR0-H-mode1), the slab layout of both
+// construction (Gaussian, 3DR0-H, 3DR0-H-mode1, and rectangular 3DR0-3DR0-H
+// between two grids with both block-tree recursions), the slab layout of both
 // passes in both precisions, adjoint and several work-item sizes with every
 // export, the block-diagonal composite, and full-block extraction with and
 // without a selection.  Built with g++ and run by tests/test_host_sanitizers.py;
@@ -23,6 +24,30 @@ static bwh_hmat* grid(int n, const char* kernel) {
   bwh_hmat* h = nullptr;
   CK(bwh_create(kernel, coor.data(), (n + 1) * (n + 1), conn.data(), n * n, props, 2, 4, 0, 0, &h));
   CK(bwh_build_hmat(h, 64, 3.0, 1e-6));
+  return h;
+}
+
+// n x n source grid in z = 0 against an m x m receiver grid lifted to z = 0.2
+static bwh_hmat* rect(int n, int m, int homogeneous) {
+  std::vector<double> cs, cr; std::vector<int64_t> ks, kr;
+  auto mesh = [](int k, double z, std::vector<double>& c, std::vector<int64_t>& q) {
+    for (int j = 0; j <= k; ++j) for (int i = 0; i <= k; ++i) {
+      c.push_back(-1.0 + 2.0 * i / k); c.push_back(-1.0 + 2.0 * j / k); c.push_back(z); }
+    for (int j = 0; j < k; ++j) for (int i = 0; i < k; ++i) {
+      int64_t a = j * (k + 1) + i; q.insert(q.end(), {a, a + 1, a + k + 2, a + k + 1}); }
+  };
+  mesh(n, 0.0, cs, ks);
+  mesh(m, 0.2, cr, kr);
+  double props[2] = {1.0, 0.25};
+  bwh_hmat* h = nullptr;
+  CK(bwh_create_rect("3DR0-3DR0-H", cs.data(), (n + 1) * (n + 1), ks.data(), n * n, cr.data(),
+                     (m + 1) * (m + 1), kr.data(), m * m, props, 2, 4, homogeneous, -1, &h));
+  CK(bwh_build_hmat(h, 16, 3.0, 1e-6));
+  int64_t r = 0, c = 0;
+  CK(bwh_shape(h, &r, &c));
+  std::vector<double> pts(3 * r), blk(9);
+  CK(bwh_receiver_collocation_points(h, pts.data()));
+  CK(bwh_influence(h, n * n - 1, m * m - 1, blk.data()));
   return h;
 }
 
@@ -73,10 +98,13 @@ int main() {
   bwh_hmat* a = grid(24, "3DR0-H");
   bwh_hmat* b = grid(13, "3DR0-H");
   bwh_hmat* m = grid(20, "3DR0-H-mode1");
-  bw_flat_hmat fg, fa, fb, fm;
+  bwh_hmat* r0 = rect(20, 13, 0);
+  bwh_hmat* r1 = rect(16, 11, 1);
+  bw_flat_hmat fg, fa, fb, fm, fr0, fr1;
   CK(bwh_flat(g, &fg)); CK(bwh_flat(a, &fa)); CK(bwh_flat(b, &fb)); CK(bwh_flat(m, &fm));
+  CK(bwh_flat(r0, &fr0)); CK(bwh_flat(r1, &fr1));
   int idx = 0;
-  for (const bw_flat_hmat* f : {&fg, &fa, &fb, &fm}) {
+  for (const bw_flat_hmat* f : {&fg, &fa, &fb, &fm, &fr0, &fr1}) {
     std::printf("matrix %d: dof %d, %ld x %ld points, %ld FR, %ld LR blocks\n", idx++, f->dof,
                 (long)f->n_row_pts, (long)f->n_col_pts, (long)f->n_fr, (long)f->n_lr);
     layouts(&f, 1);
@@ -94,7 +122,7 @@ int main() {
   const bw_flat_hmat* mixed[2] = {&fa, &fm};  // different dof: must fail cleanly
   bw_layout* lay = nullptr;
   if (bw_layout_build_multi(mixed, 2, BW_PREC_FP64, 0, 0, &lay) == 0) { std::printf("FAIL mixed dof accepted\n"); return 1; }
-  bwh_free(g); bwh_free(a); bwh_free(b); bwh_free(m);
+  bwh_free(g); bwh_free(a); bwh_free(b); bwh_free(m); bwh_free(r0); bwh_free(r1);
   std::printf("host sanitizer driver ok\n");
   return 0;
 }

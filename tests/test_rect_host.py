@@ -153,3 +153,19 @@ def test_python_class_before_build():
     with pytest.raises(ValueError, match="Invalid kernel"):
         BEMatrixRectangular("3DT0-H", cs, ks, cr, kr, np.array([1.0, 0.25]),
                             directly_build=False)
+
+
+def test_store_round_trip(tmp_path):
+    """A rectangular operator saved and loaded (bigwham_dev_amd.store): both
+    permutations and every block come back, so the oracle's product is the
+    same to the bit."""
+    from bigwham_dev_amd import store
+    h, leaf, _ = _host("3DR0-3DR0-H", True, small=True)
+    h.build(leaf, 3.0, 1e-6)
+    store.save(h, tmp_path / "rect.hmat")
+    g = store.load(tmp_path / "rect.hmat")
+    f, fg = h.flat(), g.flat()
+    assert (fg.n_row_pts, fg.n_col_pts) == (f.n_row_pts, f.n_col_pts)
+    x = np.random.default_rng(4).uniform(-1, 1, h.shape[1])
+    assert np.array_equal(O.matvec_original(fg, x, n_threads=1),
+                          O.matvec_original(f, x, n_threads=1))

@@ -41,3 +41,19 @@ def pair(kernel: str, small: bool = False):
 
 
 KERNELS = ["2DS0-2DS0-H", "3DR0-3DR0-H", "3DT0-3DT0-H", "3DR0-3DR0-H-mode1"]
+
+
+def edge_pairs():
+    """Degenerate sides: one source element against a grid of receivers and
+    a grid of sources against one receiver (kernel, coor_src, conn_src,
+    coor_rec, conn_rec, leaf)."""
+    one, k1 = rect_grid(1, half=0.3)
+    grid, kg = rect_grid(10)
+    lifted = grid + np.array([0.0, 0.0, 0.2])
+    one_up = one + np.array([0.05, -0.02, 0.15])
+    s1, q1 = segment_line(1, L=0.2)
+    sl, ql = segment_line(50)
+    return {"one_source": ("3DR0-3DR0-H", one, k1, lifted, kg, 8),
+            "one_receiver": ("3DR0-3DR0-H", grid, kg, one_up, k1, 8),
+            "segment_one_source": ("2DS0-2DS0-H", s1, q1, sl + np.array([0.0, 0.1]), ql, 8),
+            "segment_one_receiver": ("2DS0-2DS0-H", sl, ql, s1 + np.array([0.03, 0.1]), q1, 8)}

@@ -10,7 +10,7 @@ import pytest
 
 import oracle as O
 from bigwham_dev_amd.bigwham4py import BEMatrixRectangular
-from rect_meshes import KERNELS, pair
+from rect_meshes import KERNELS, edge_pairs, pair
 
 pytestmark = pytest.mark.gpu
 
@@ -90,3 +90,34 @@ def test_full_blocks(M):
         i, j = coo.row[k], coo.col[k]
         blk = M.H_.influence(j // d, i // d)
         assert blk[i % d, j % d] == coo.data[k]
+
+
+@pytest.mark.parametrize("case", list(edge_pairs()))
+def test_degenerate_sides(gpu, case):
+    """One source element against a grid of receivers, and the reverse:
+    single vectors and 5 / 32 right-hand sides, H and H^T, both precisions."""
+    import torch
+    kernel, cs, ks, cr, kr, leaf = edge_pairs()[case]
+    print(f"rectangular {case}: {len(ks)} sources, {len(kr)} receivers", flush=True)
+    m = BEMatrixRectangular(kernel, cs, ks, cr, kr, np.array([1.0, 0.25]), max_leaf_size=leaf,
+                            eps_aca=1e-6, verbose=False, device=0)
+    try:
+        f = m.H_.flat()
+        rng = np.random.default_rng(10)
+        x = rng.uniform(-1, 1, m.shape[1])
+        u = rng.uniform(-1, 1, m.shape[0])
+        assert rel(m @ x, O.matvec_original(f, x)) <= 1e-12
+        assert rel(m.T @ u, O.rmatvec_original(f, u)) <= 1e-12
+        for k in (5, 32):
+            for precision, tol in (("fp64", 1e-12), ("fp32", 1e-5)):
+                X = rng.uniform(-1, 1, (m.shape[1], k))
+                Y = m.dev_.matvec_batched(torch.from_numpy(X).to(gpu), original=True,
+                                          precision=precision).cpu().numpy()
+                U = rng.uniform(-1, 1, (m.shape[0], k))
+                V = m.dev_.rmatvec_batched(torch.from_numpy(U).to(gpu), original=True,
+                                           precision=precision).cpu().numpy()
+                for j in (0, k - 1):
+                    assert rel(Y[:, j], O.matvec_original(f, X[:, j])) <= tol
+                    assert rel(V[:, j], O.rmatvec_original(f, U[:, j])) <= tol
+    finally:
+        m.dev_.close()

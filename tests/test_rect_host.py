@@ -21,7 +21,8 @@ import pytest
 
 import oracle as O
 from bigwham_dev_amd import BigWhamError, HostHmat
-from rect_meshes import KERNELS, pair
+from layout_emu import LayoutDump
+from rect_meshes import KERNELS, edge_pairs, pair
 
 ROOT = Path(__file__).resolve().parents[1]
 REF = ROOT / "oracle" / "_ref" / "libbwref.so"
@@ -169,3 +170,32 @@ def test_store_round_trip(tmp_path):
     x = np.random.default_rng(4).uniform(-1, 1, h.shape[1])
     assert np.array_equal(O.matvec_original(fg, x, n_threads=1),
                           O.matvec_original(f, x, n_threads=1))
+
+
+@pytest.mark.parametrize("case", ["3DR0-3DR0-H", "3DT0-3DT0-H"] + list(edge_pairs()))
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("adjoint", [False, True])
+def test_layout_replay(case, precision, adjoint):
+    """The device layout of a rectangular operator between two meshes, its
+    degenerate sides included (one source or one receiver element): every
+    address the kernels form stays in its buffer, and a numpy replay of the
+    kernels equals the oracle in both numberings."""
+    if case in KERNELS:
+        cs, ks, cr, kr, leaf = pair(case, small=True)
+        kernel = case
+    else:
+        kernel, cs, ks, cr, kr, leaf = edge_pairs()[case]
+    h = HostHmat.from_meshes(kernel, cs, ks, cr, kr, [1.0, 0.25])
+    h.build(leaf, 3.0, 1e-6)
+    f = h.flat()
+    lay = LayoutDump(f, precision, adjoint, 4096)
+    lay.check_bounds()
+    lay.check_xcd_cover()
+    x = np.random.default_rng(6).uniform(-1, 1, lay.n_in)
+    tol = 1e-13 if precision == "fp64" else 1e-6
+    ref = (O.rmatvec if adjoint else O.matvec)(f, x)
+    assert np.linalg.norm(lay.replay(x) - ref) <= tol * np.linalg.norm(ref)
+    p0, p1 = h.permutation_receivers(), h.permutation()
+    pin, pout = (p0, p1) if adjoint else (p1, p0)
+    ref = (O.rmatvec_original if adjoint else O.matvec_original)(f, x)
+    assert np.linalg.norm(lay.replay(x, pin, pout) - ref) <= tol * np.linalg.norm(ref)

@@ -10,15 +10,16 @@
 // Per block, in double precision, without forming an m x n matrix:
 //   scale:   A = Abar Da, B = Db Bhat with unit columns of Abar and unit rows
 //            of Bhat (Da, Db diagonal; terms with a zero column or row drop);
-//   Gram:    Abar^T Abar = Wa La Wa^T and Bhat Bhat^T = Wb Lb Wb^T (Jacobi
-//            eigensolver; eigenvalues below 1e-14 of the largest are null
-//            directions and dropped), so Abar = Qa La^1/2 Wa^T and
-//            Bhat^T = Qb Lb^1/2 Wb^T with orthonormal Qa, Qb;
-//   core:    C = La^1/2 Wa^T (Da Db) Wb Lb^1/2 = U S V^T (one-sided Jacobi);
+//   Gram:    Abar^T Abar = Ra^T Ra and Bhat Bhat^T = Rb^T Rb (Cholesky with
+//            diagonal pivoting; directions whose remaining diagonal is below
+//            1e-14 of the largest are null directions and dropped), so
+//            Abar = Qa Ra and Bhat^T = Qb Rb with orthonormal Qa, Qb and right
+//            inverses Ra Rainv = I, Rb Rbinv = I;
+//   core:    C = Ra (Da Db) Rb^T = U S V^T (one-sided Jacobi);
 //   cut:     the smallest k whose discarded singular values have a root sum
 //            of squares <= tol * ||S||_F;
-//   factors: A' = A Ta, Ta = Da^-1 Wa La^-1/2 U_k S_k   (r x k)
-//            B' = Tb B, Tb = V_k^T Lb^-1/2 Wb^T Db^-1   (k x r)
+//   factors: A' = A Ta, Ta = Da^-1 Rainv U_k S_k   (r x k)
+//            B' = Tb B, Tb = V_k^T Rbinv^T Db^-1   (k x r)
 // so that A' B' = Qa U_k S_k V_k^T Qb^T.  Blocks whose rank does not drop
 // keep their original factors.  Work is O((m + n) r^2) per block, spread
 // over host threads, largest blocks first.
@@ -38,50 +39,6 @@
 namespace bwl {
 namespace {
 
-// Cyclic Jacobi eigensolver of a symmetric p x p matrix S (row-major,
-// overwritten).  On return lam[i] are the eigenvalues and column i of W
-// (row-major p x p) the eigenvector of lam[i].
-void jacobi_eigen(int p, std::vector<double>& S, std::vector<double>& lam, std::vector<double>& W) {
-  W.assign(static_cast<size_t>(p) * p, 0.0);
-  for (int i = 0; i < p; ++i) W[static_cast<size_t>(i) * p + i] = 1.0;
-  auto at = [&](int i, int j) -> double& { return S[static_cast<size_t>(i) * p + j]; };
-  for (int sweep = 0; sweep < 60; ++sweep) {
-    double off = 0.0, diag = 0.0;
-    for (int i = 0; i < p; ++i) {
-      diag += at(i, i) * at(i, i);
-      for (int j = i + 1; j < p; ++j) off += at(i, j) * at(i, j);
-    }
-    if (off <= 1e-32 * diag || off == 0.0) break;
-    for (int a = 0; a < p - 1; ++a)
-      for (int b = a + 1; b < p; ++b) {
-        const double sab = at(a, b);
-        if (std::fabs(sab) <= 1e-300) continue;
-        const double theta = (at(b, b) - at(a, a)) / (2.0 * sab);
-        const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
-        const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
-        // S <- J^T S J on rows / columns a, b
-        for (int k = 0; k < p; ++k) {
-          const double ska = at(k, a), skb = at(k, b);
-          at(k, a) = c * ska - s * skb;
-          at(k, b) = s * ska + c * skb;
-        }
-        for (int k = 0; k < p; ++k) {
-          const double sak = at(a, k), sbk = at(b, k);
-          at(a, k) = c * sak - s * sbk;
-          at(b, k) = s * sak + c * sbk;
-        }
-        for (int k = 0; k < p; ++k) {
-          double* w = &W[static_cast<size_t>(k) * p];
-          const double wa = w[a], wb = w[b];
-          w[a] = c * wa - s * wb;
-          w[b] = s * wa + c * wb;
-        }
-      }
-  }
-  lam.resize(p);
-  for (int i = 0; i < p; ++i) lam[i] = at(i, i);
-}
-
 // One-sided Jacobi SVD of X (rows x cols, column-major, cols <= rows,
 // overwritten by U S).  On return sig[j] (unsorted), U (column-major rows x
 // cols, unit columns where sig > 0) and V (column-major cols x cols).
@@ -89,23 +46,33 @@ void jacobi_svd(int rows, int cols, std::vector<double>& X, std::vector<double>&
                 std::vector<double>& V) {
   V.assign(static_cast<size_t>(cols) * cols, 0.0);
   for (int i = 0; i < cols; ++i) V[static_cast<size_t>(i) * cols + i] = 1.0;
+  // squared column norms: recomputed at the start of every sweep and carried
+  // through each rotation exactly as the rotation changes them
+  // (|x_i'|^2 = |x_i|^2 - t gamma, |x_j'|^2 = |x_j|^2 + t gamma for the t
+  // below), so a pair costs one dot product instead of three
+  std::vector<double> nrm(cols);
   for (int sweep = 0; sweep < 60; ++sweep) {
     bool rotated = false;
+    for (int j = 0; j < cols; ++j) {
+      const double* xj = &X[static_cast<size_t>(j) * rows];
+      double nn = 0;
+      for (int k = 0; k < rows; ++k) nn += xj[k] * xj[k];
+      nrm[j] = nn;
+    }
     for (int i = 0; i < cols - 1; ++i)
       for (int j = i + 1; j < cols; ++j) {
         double* xi = &X[static_cast<size_t>(i) * rows];
         double* xj = &X[static_cast<size_t>(j) * rows];
-        double alpha = 0, beta = 0, gamma = 0;
-        for (int k = 0; k < rows; ++k) {
-          alpha += xi[k] * xi[k];
-          beta += xj[k] * xj[k];
-          gamma += xi[k] * xj[k];
-        }
+        const double alpha = nrm[i], beta = nrm[j];
+        double gamma = 0;
+        for (int k = 0; k < rows; ++k) gamma += xi[k] * xj[k];
         if (std::fabs(gamma) <= 1e-15 * std::sqrt(alpha * beta) || gamma == 0.0) continue;
         rotated = true;
         const double zeta = (beta - alpha) / (2.0 * gamma);
         const double t = (zeta >= 0 ? 1.0 : -1.0) / (std::fabs(zeta) + std::sqrt(1.0 + zeta * zeta));
         const double c = 1.0 / std::sqrt(1.0 + t * t), s = c * t;
+        nrm[i] = alpha - t * gamma;
+        nrm[j] = beta + t * gamma;
         for (int k = 0; k < rows; ++k) {
           const double a = xi[k], b = xj[k];
           xi[k] = c * a - s * b;
@@ -132,30 +99,68 @@ void jacobi_svd(int rows, int cols, std::vector<double>& X, std::vector<double>&
   }
 }
 
-// Symmetric factor of a Gram matrix: keeps the eigenpairs above 1e-14 of
-// the largest eigenvalue.  R (p x q, row-major) = L^1/2 W^T and
-// Rinv (q x p, row-major) = W L^-1/2 over the kept pairs.
+// Factor of a Gram matrix G (q x q, row-major, overwritten) by Cholesky with
+// diagonal pivoting: at each step the largest remaining diagonal entry is
+// eliminated, and the factorisation stops once every remaining one is below
+// 1e-14 of the largest diagonal entry of G (those directions hold less than
+// 1e-7 of a column's norm and are dropped, as the null directions of the
+// Gram matrix).  With the pivot order piv and the p x p leading lower
+// triangle L11 of the pivoted factor L (P^T G P ~ L L^T), returns
+// R (p x q, row-major) = (P L)^T, so that G ~ R^T R, and
+// Rinv (q x p, row-major) = P [L11^-T; 0], so that R Rinv = I: the columns
+// piv[0..p) of the factor are an orthonormal basis times L11^T.
 void gram_factor(int q, std::vector<double>& G, std::vector<double>& R, std::vector<double>& Rinv,
                  int* p_out) {
-  std::vector<double> lam, W;
-  jacobi_eigen(q, G, lam, W);
-  double lmax = 0;
-  for (double l : lam) lmax = std::max(lmax, l);
-  std::vector<int> keep;
-  for (int i = 0; i < q; ++i)
-    if (lam[i] > 1e-14 * lmax && lam[i] > 0) keep.push_back(i);
-  const int p = static_cast<int>(keep.size());
-  R.assign(static_cast<size_t>(p) * q, 0.0);
-  Rinv.assign(static_cast<size_t>(q) * p, 0.0);
-  for (int a = 0; a < p; ++a) {
-    const int i = keep[a];
-    const double sl = std::sqrt(lam[i]);
-    for (int j = 0; j < q; ++j) {
-      const double w = W[static_cast<size_t>(j) * q + i];  // component j of eigenvector i
-      R[static_cast<size_t>(a) * q + j] = sl * w;
-      Rinv[static_cast<size_t>(j) * p + a] = w / sl;
+  auto at = [&](int i, int j) -> double& { return G[static_cast<size_t>(i) * q + j]; };
+  std::vector<int> piv(q);
+  for (int i = 0; i < q; ++i) piv[i] = i;
+  double dmax = 0;
+  for (int i = 0; i < q; ++i) dmax = std::max(dmax, at(i, i));
+  const double floor = 1e-14 * dmax;
+  // L in pivoted order, row-major q x q (rows: pivoted positions)
+  std::vector<double> L(static_cast<size_t>(q) * q, 0.0);
+  auto l = [&](int i, int j) -> double& { return L[static_cast<size_t>(i) * q + j]; };
+  // d[i]: remaining diagonal of the Schur complement at pivoted position i
+  std::vector<double> d(q);
+  for (int i = 0; i < q; ++i) d[i] = at(i, i);
+  int p = 0;
+  for (; p < q; ++p) {
+    int best = p;
+    for (int i = p + 1; i < q; ++i)
+      if (d[i] > d[best]) best = i;
+    if (!(d[best] > floor)) break;
+    if (best != p) {
+      std::swap(piv[p], piv[best]);
+      std::swap(d[p], d[best]);
+      for (int j = 0; j < p; ++j) std::swap(l(p, j), l(best, j));
+    }
+    const double lpp = std::sqrt(d[p]);
+    l(p, p) = lpp;
+    for (int i = p + 1; i < q; ++i) {
+      double v = at(piv[i], piv[p]);
+      for (int j = 0; j < p; ++j) v -= l(i, j) * l(p, j);
+      v /= lpp;
+      l(i, p) = v;
+      d[i] -= v * v;
     }
   }
+  R.assign(static_cast<size_t>(p) * q, 0.0);
+  for (int a = 0; a < p; ++a)
+    for (int i = a; i < q; ++i) R[static_cast<size_t>(a) * q + piv[i]] = l(i, a);
+  // L11^-1 (lower triangular) by forward substitution, column by column
+  std::vector<double> Linv(static_cast<size_t>(p) * p, 0.0);
+  for (int c = 0; c < p; ++c) {
+    Linv[static_cast<size_t>(c) * p + c] = 1.0 / l(c, c);
+    for (int i = c + 1; i < p; ++i) {
+      double v = 0;
+      for (int j = c; j < i; ++j) v -= l(i, j) * Linv[static_cast<size_t>(j) * p + c];
+      Linv[static_cast<size_t>(i) * p + c] = v / l(i, i);
+    }
+  }
+  Rinv.assign(static_cast<size_t>(q) * p, 0.0);
+  for (int i = 0; i < p; ++i)
+    for (int a = 0; a <= i; ++a)  // (L11^-T)[a][i] = Linv[i][a]
+      Rinv[static_cast<size_t>(piv[a]) * p + i] = Linv[static_cast<size_t>(i) * p + a];
   *p_out = p;
 }
 

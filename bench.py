@@ -77,9 +77,10 @@ def load_traffic(config):
     return out, doc.get("fp32_rank_tolerance", 0.0)
 
 
-def load_call_traffic(path, tol32):
-    """HBM bytes per batched FP32 call from a PMC summary (None if absent or
-    measured on an FP32 copy of another recompression tolerance)."""
+def load_call_traffic(path, tol32, precision="fp32"):
+    """HBM bytes per batched call of one precision from a PMC summary (None if
+    absent or, for FP32, measured on a copy of another recompression
+    tolerance)."""
     import re
     if not (ROOT / path).exists():
         return None
@@ -87,9 +88,9 @@ def load_call_traffic(path, tol32):
     if not m:
         return None
     doc = json.loads(m.group(1))
-    if doc.get("fp32_rank_tolerance", 0.0) != tol32:
+    if precision == "fp32" and doc.get("fp32_rank_tolerance", 0.0) != tol32:
         return None
-    return doc.get("traffic_bytes_per_call", {}).get("fp32")
+    return doc.get("traffic_bytes_per_call", {}).get(precision)
 
 
 def log(*a):
@@ -656,7 +657,12 @@ def bench_batching(args, cfg):
         "single_rhs": {"fp32_ms": round(t_s32 * 1e3, 4), "fp64_ms": round(t_s64 * 1e3, 4),
                        "batched_speedup_vs_32_fp32_calls": round(k * t_s32 / t_b32, 2)},
         "fp64_batched": {"ms_per_call": round(t_b64 * 1e3, 4), "rhs_per_s": round(k / t_b64, 1),
-                         "roofline": roof(t_b64, 8, 78.6, bytes_b64, flops),
+                         "roofline": {**roof(t_b64, 8, 78.6, bytes_b64, flops),
+                                      "traffic": load_call_traffic(
+                                          "profiles/hbm_traffic_3dr0_150k_dof_32rhs.md", tol32,
+                                          "fp64"),
+                                      "traffic_source":
+                                          "profiles/hbm_traffic_3dr0_150k_dof_32rhs.md"},
                          "speedup_vs_32_fp64_calls": round(k * t_s64 / t_b64, 2)},
         "parity": {"batched_fp32_rel_err_vs_fp64": err, "batched_fp64_rel_err_vs_fp64": err64},
         "vs_dense_operator": vs_dense,

@@ -199,3 +199,22 @@ def test_layout_replay(case, precision, adjoint):
     pin, pout = (p0, p1) if adjoint else (p1, p0)
     ref = (O.rmatvec_original if adjoint else O.matvec_original)(f, x)
     assert np.linalg.norm(lay.replay(x, pin, pout) - ref) <= tol * np.linalg.norm(ref)
+
+
+def test_fixed_rank_and_homogeneous():
+    """fixed_rank reaches ACA on a rectangular operator as on a square one:
+    every low-rank block stops after fixed_rank pivots (dof * fixed_rank
+    columns).  On this pair the size-conserving block tree makes every
+    low-rank block leaf x leaf points (checked), so fixed_rank = leaf
+    represents them exactly."""
+    cs, ks, cr, kr, leaf = pair("3DR0-3DR0-H", small=True)
+    for fixed in (4, leaf):
+        h = HostHmat.from_meshes("3DR0-3DR0-H", cs, ks, cr, kr, [1.0, 0.25],
+                                 homogeneous_size_pattern=True, fixed_rank=fixed)
+        h.build(leaf, 3.0, 1e-6)
+        lp = h.lr_pattern()
+        assert lp.shape[1] > 0 and lp[5].max() <= 3 * fixed
+        assert np.all(lp[3] - lp[1] == leaf) and np.all(lp[4] - lp[2] == leaf)
+    D = h.dense(original=True)
+    x = np.random.default_rng(2).uniform(-1, 1, D.shape[1])
+    assert np.linalg.norm(O.matvec_original(h.flat(), x) - D @ x) <= 1e-12 * np.linalg.norm(D @ x)

@@ -109,6 +109,8 @@ class HostHmat:
 
     @property
     def size(self) -> int:
+        """dof x the mesh's collocation points: the order of a square
+        operator (the column count of a rectangular one; see `shape`)."""
         dof, _, n = self.info()
         return dof * n
 

@@ -825,13 +825,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
     }
     return;
   }
-  // partial sums of split slabs are added in order by k_reduce_b
+  // partial sums of split slabs are added in order by k_reduce_b; they are
+  // read once, by it, so they are written (and read) nontemporally: about 1 %
+  // of a 32-RHS call on a 150 528-DoF operator, same values
   float* pp = part + static_cast<int64_t>(wi.part_off) * kRhs;
 #pragma unroll
   for (int j = 0; j < 16; ++j) {
     const int row = row_of(j);
-    if (row < R) pp[row * kRhs + n] = acc0[j];
-    if (two && 32 + row < R) pp[(32 + row) * kRhs + n] = acc1[j];
+    if (row < R) __builtin_nontemporal_store(acc0[j], &pp[row * kRhs + n]);
+    if (two && 32 + row < R) __builtin_nontemporal_store(acc1[j], &pp[(32 + row) * kRhs + n]);
   }
 }
 
@@ -949,7 +951,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(D > 3 ? 3 : 
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int row = 16 * t + 4 * kq + r;
-          if (row < R) pp[row * kRhs + 16 * h + i16] = acc[t][h][r];  // added in order by k_reduce_b
+          if (row < R)  // added in order by k_reduce_b
+            __builtin_nontemporal_store(acc[t][h][r], &pp[row * kRhs + 16 * h + i16]);
         }
     }
     return;
@@ -1047,7 +1050,7 @@ __global__ __launch_bounds__(256) void k_reduce_b(const bwl::Reduce* __restrict_
                                                   const int32_t* __restrict__ rowmap) {
   // one workgroup per split slab; a thread sums one 16-byte vector of
   // right-hand sides of one row over the parts, in part order, four parts in
-  // flight
+  // flight (nontemporal loads: each partial row is read once)
   typedef typename Vec<T>::type VT;
   constexpr int VW = Vec<T>::V;
   const int r = blockIdx.x;
@@ -1061,16 +1064,16 @@ __global__ __launch_bounds__(256) void k_reduce_b(const bwl::Reduce* __restrict_
     VT sum = {};
     int q = 0;
     for (; q + 4 <= rd.nparts; q += 4) {
-      const VT a = base_p[(q + 0) * stride + e];
-      const VT b = base_p[(q + 1) * stride + e];
-      const VT c = base_p[(q + 2) * stride + e];
-      const VT d = base_p[(q + 3) * stride + e];
+      const VT a = __builtin_nontemporal_load(&base_p[(q + 0) * stride + e]);
+      const VT b = __builtin_nontemporal_load(&base_p[(q + 1) * stride + e]);
+      const VT c = __builtin_nontemporal_load(&base_p[(q + 2) * stride + e]);
+      const VT d = __builtin_nontemporal_load(&base_p[(q + 3) * stride + e]);
       sum += a;
       sum += b;
       sum += c;
       sum += d;
     }
-    for (; q < rd.nparts; ++q) sum += base_p[q * stride + e];
+    for (; q < rd.nparts; ++q) sum += __builtin_nontemporal_load(&base_p[q * stride + e]);
     const int row = e / (kRhs / VW);
     const int n0 = (e % (kRhs / VW)) * VW;
     if (PASS == 1) {
@@ -1231,7 +1234,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(D > 4 ? 2 : 
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int row = 16 * t + k + 4 * r;
-          if (row < R) pp[row * kRhs + 16 * c + i16] = acc[t][c][r];  // added in order by k_reduce_b
+          if (row < R)  // added in order by k_reduce_b
+            __builtin_nontemporal_store(acc[t][c][r], &pp[row * kRhs + 16 * c + i16]);
         }
     }
     return;
